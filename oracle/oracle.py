@@ -1,0 +1,203 @@
+"""ctypes wrapper over liboracle.so -- CPU ORACLE, test infrastructure only.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may import this
+module, and only as the checker (or the timed CPU baseline).  The product package
+(gem_amd) never imports it.
+
+Also loads the reference's own turbo_tricublin.c when `make ref` has compiled it into
+oracle/_ref/ (only where the reference tree exists), to pin the restatement.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+REF_AVX2 = os.path.join(HERE, "_ref", "libtricublin_ref_avx2.so")
+REF_SCALAR = os.path.join(HERE, "_ref", "libtricublin_ref_scalar.so")
+
+ORD_AVX2, ORD_SCALAR = 0, 1
+_lib = None
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-C", HERE, "liboracle.so"])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = C.CDLL(LIB)
+        vp, i, l, d = C.c_void_p, C.c_int, C.c_long, C.c_double
+        L.ora_ztab_new.restype = vp
+        L.ora_ztab_new.argtypes = [vp, i, i, i, i, i]
+        L.ora_ztab_free.argtypes = [vp]
+        L.ora_ztab_get.argtypes = [vp, vp, vp]
+        L.ora_vcoef.argtypes = [vp, vp, i, i, i, vp, vp, vp]
+        L.ora_zyx1_m_n.argtypes = [vp, vp, i, vp, i, vp, l, i, i, i]
+        L.ora_mono_m_n.argtypes = [vp, vp, vp, vp, vp, i, vp, i, vp, l, i, i, i]
+        L.ora_zyx3_n.argtypes = [vp, vp, i, vp, i, vp, l, i, i]
+        L.ora_grid_new.restype = vp
+        L.ora_grid_new.argtypes = [vp]
+        L.ora_grid_free.argtypes = [vp]
+        L.ora_grid_info.argtypes = [vp, vp, vp]
+        L.ora_grid_table.argtypes = [vp, i, vp]
+        L.ora_level_search.argtypes = [vp, i, l, vp, vp]
+        L.ora_zindx.argtypes = [vp, l, vp, vp]
+        L.ora_traject.argtypes = [vp, i, i, d, i, d, d, i, i] + [vp] * 6 + [vp] * 10
+        L.ora_almost_zero.restype = i
+        L.ora_almost_zero.argtypes = [d]
+        L.ora_fsum.restype = d
+        L.ora_fsum.argtypes = [vp, l]
+        L.ora_bermejo_conde.argtypes = [i, vp, i, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]
+        L.ora_mass.restype = d
+        L.ora_mass.argtypes = [i, vp, vp, vp, vp]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p) if a is not None else None
+
+
+def _ptrs(arrs, offsets):
+    """Array of pointers to element offsets[k] of arrs[k] (the f(1,1,1) address)."""
+    out = (C.c_void_p * len(arrs))()
+    for k, (a, o) in enumerate(zip(arrs, offsets)):
+        out[k] = a.ctypes.data + int(o) * a.itemsize
+    return out
+
+
+class Ztab:
+    """Oracle vertical descriptor (Vsearch_setup_plus restatement)."""
+
+    def __init__(self, levels, ni, nj, offi=0, offj=0):
+        self.levels = np.ascontiguousarray(levels, dtype=np.float64)
+        self.nk, self.ni, self.nj, self.offi, self.offj = len(levels), ni, nj, offi, offj
+        self.h = lib().ora_ztab_new(_p(self.levels), self.nk, ni, nj, offi, offj)
+
+    def tables(self):
+        z = np.zeros(self.nk)
+        ocz = np.zeros(4 * max(self.nk, 4))
+        lib().ora_ztab_get(self.h, _p(z), _p(ocz))
+        return z, ocz
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.ora_ztab_free(self.h)
+            self.h = None
+
+
+def _pdt(pos):
+    return 1 if pos.dtype == np.float32 else 0
+
+
+# The reference's production build (gcc -O2 -mavx2 -mfma) contracts the absolute-z
+# expression of Vcoef_pxyz4_inline (turbo_tricublin.c:438) into fma(dz, z1, (1-dz)*z0);
+# the portable build has no FMA.  zabs_fma=None picks the one matching `order`.
+def _zf(order, zabs_fma):
+    if zabs_fma is None:
+        return 1 if order == ORD_AVX2 else 0
+    return zabs_fma
+
+
+def vcoef(zt, pos, zabs_fma=1):
+    pos = np.ascontiguousarray(pos)
+    n = pos.size // 3
+    ijk = np.zeros(n, np.int32)
+    zl = np.zeros(n, np.int32)
+    coef = np.zeros((n, 20))
+    lib().ora_vcoef(zt.h, _p(pos), _pdt(pos), n, zabs_fma, _p(ijk), _p(zl), _p(coef))
+    return ijk, zl, coef
+
+
+def zyx1_m_n(zt, fields, base_off, pos, order=ORD_AVX2, zabs_fma=None):
+    """fields: list of flat arrays (same dtype); base_off: element offset of f(1,1,1)."""
+    pos = np.ascontiguousarray(pos)
+    n = pos.size // 3
+    dt = fields[0].dtype
+    d = np.zeros(len(fields) * n, dt)
+    fs = _ptrs(fields, [base_off] * len(fields))
+    lib().ora_zyx1_m_n(_p(d), fs, 1 if dt == np.float32 else 0, _p(pos), _pdt(pos), zt.h, n, len(fields),
+                       order, _zf(order, zabs_fma))
+    return d.reshape(len(fields), n)
+
+
+def mono_m_n(zt, fields, base_off, pos, order=ORD_AVX2, zabs_fma=None):
+    pos = np.ascontiguousarray(pos)
+    n = pos.size // 3
+    dt = fields[0].dtype
+    outs = [np.zeros(len(fields) * n, dt) for _ in range(4)]
+    fs = _ptrs(fields, [base_off] * len(fields))
+    lib().ora_mono_m_n(*[_p(o) for o in outs], fs, 1 if dt == np.float32 else 0, _p(pos), _pdt(pos), zt.h, n,
+                       len(fields), order, _zf(order, zabs_fma))
+    return [o.reshape(len(fields), n) for o in outs]
+
+
+def zyx3_n(zt, f3, base_off, pos, order=ORD_AVX2, zabs_fma=None):
+    pos = np.ascontiguousarray(pos)
+    n = pos.size // 3
+    d = np.zeros(3 * n, f3.dtype)
+    ptr = f3.ctypes.data + int(base_off) * f3.itemsize
+    lib().ora_zyx3_n(_p(d), C.c_void_p(ptr), 1 if f3.dtype == np.float32 else 0, _p(pos), _pdt(pos), zt.h, n,
+                     order, _zf(order, zabs_fma))
+    return d.reshape(n, 3)
+
+
+def almost_zero(a):
+    return bool(lib().ora_almost_zero(float(a)))
+
+
+def fsum(v):
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    return lib().ora_fsum(_p(v), v.size)
+
+
+# ------------------------------------------------------------------ reference (_ref)
+class RefKernel:
+    """The reference's turbo_tricublin.c compiled from its own source (oracle/_ref)."""
+
+    def __init__(self, which="avx2"):
+        path = REF_AVX2 if which == "avx2" else REF_SCALAR
+        if not os.path.exists(path):
+            raise FileNotFoundError(path)
+        L = C.CDLL(path)
+        vp, i = C.c_void_p, C.c_int
+        L.Vsearch_setup_plus.restype = vp
+        L.Vsearch_setup_plus.argtypes = [vp, i, i, i, i, i]
+        for name in ("Tricublin_zyx1_m_n", "Tricublin_zyx1_n_m"):
+            getattr(L, name).argtypes = [vp, vp, vp, vp, i, i]
+        L.Tricublin_zyx1_n.argtypes = [vp, vp, vp, vp, i]
+        L.Tricublin_mono_zyx_m_n.argtypes = [vp, vp, vp, vp, vp, vp, vp, i, i]
+        L.Tricublin_zyx3_n.argtypes = [vp, vp, vp, vp, i]
+        self.L = L
+
+    def setup(self, levels, ni, nj, offi=0, offj=0):
+        self._lev = np.ascontiguousarray(levels, dtype=np.float64)
+        return self.L.Vsearch_setup_plus(_p(self._lev), len(levels), ni, nj, offi, offj)
+
+    def zyx1_m_n(self, lv, fields, base_off, pos):
+        pos = np.ascontiguousarray(pos, dtype=np.float32)
+        n = pos.size // 3
+        d = np.zeros(len(fields) * n, np.float32)
+        self.L.Tricublin_zyx1_m_n(_p(d), _ptrs(fields, [base_off] * len(fields)), _p(pos), lv, n, len(fields))
+        return d.reshape(len(fields), n)
+
+    def mono_m_n(self, lv, fields, base_off, pos):
+        pos = np.ascontiguousarray(pos, dtype=np.float32)
+        n = pos.size // 3
+        outs = [np.zeros(len(fields) * n, np.float32) for _ in range(4)]
+        self.L.Tricublin_mono_zyx_m_n(*[_p(o) for o in outs], _ptrs(fields, [base_off] * len(fields)), _p(pos), lv,
+                                      n, len(fields))
+        return [o.reshape(len(fields), n) for o in outs]
+
+    def zyx3_n(self, lv, f3, base_off, pos):
+        pos = np.ascontiguousarray(pos, dtype=np.float32)
+        n = pos.size // 3
+        d = np.zeros(3 * n, np.float32)
+        self.L.Tricublin_zyx3_n(_p(d), C.c_void_p(f3.ctypes.data + int(base_off) * 4), _p(pos), lv, n)
+        return d.reshape(n, 3)
