@@ -1,0 +1,132 @@
+"""Host mirror of GEM's adz_* layer (gemdyn/src/base/adz_*.F90) over libadz_hip.
+
+`AdzContext(grid)` is adz_set (adz_set.F90:16-311): it owns the SL state that the
+Fortran keeps in module adz_mem (positions, derived winds, tables) in device memory.
+Methods keep the Fortran names: adz_inittraj(), adz_traject(dt, ...).  Arguments that
+are arrays are torch device tensors.
+"""
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import _lib as L
+
+L.extra_sigs({
+    "adz_set": ("i", ["vp", "i", "i", "vp"]),
+    "adz_ctx_free": ("i", ["vp"]),
+    "adz_ctx_info": ("i", ["vp", "vp", "vp"]),
+    "adz_ctx_table": ("i", ["vp", "i", "vp"]),
+    "adz_ctx_array": ("i", ["vp", "i", "vp", "vp"]),
+    "adz_ctx_ztab": ("vp", ["vp", "i"]),
+    "adz_inittraj": ("i", ["vp", "vp"]),
+    "adz_traject": ("i", ["vp", "d", "i", "d", "d"] + ["vp"] * 6 + ["vp"]),
+})
+
+ADZ_A_PXYZM, ADZ_A_WPXYZ, ADZ_A_UVW_DEP, ADZ_A_PM, ADZ_A_PMU, ADZ_A_PMV, ADZ_A_PT, ADZ_A_PB, ADZ_A_UVW_D, \
+    ADZ_A_WARR = range(10)
+
+INFO_INTS = ["lminx", "lmaxx", "lminy", "lmaxy", "nit", "njt", "nij", "i0", "in", "j0", "jn", "i0u", "inu", "j0v",
+             "jnv", "i0b", "inb", "j0b", "jnb", "k0", "k0t", "k0m", "pnz_m", "pnz_t"]
+INFO_DBLS = ["iminposx", "imaxposx", "iminposy", "imaxposy", "sig", "ovdzm", "ovdzt", "vw5", "vw6"]
+
+
+class AdzGrid(C.Structure):
+    """struct adz_grid (include/adz_hip.h)."""
+    _fields_ = [(n, C.c_int) for n in ("ni", "nj", "nk", "i0g", "j0g", "halox", "haloy", "west", "east", "south",
+                                       "north", "pil_w", "pil_e", "pil_s", "pil_n", "gbpil_t", "bc_lam_flux", "niu",
+                                       "njv")] + \
+              [("inv_hx", C.c_double), ("inv_hy", C.c_double), ("inv_rayt", C.c_double)] + \
+              [(n, C.c_void_p) for n in ("zm", "zt", "zx", "lat")] + \
+              [(n, C.c_double) for n in ("zmin", "zmax", "wmstar_nk", "idz_t_nkm1")]
+
+    @classmethod
+    def from_dict(cls, g):
+        s = cls()
+        keep = {}
+        for name, ctype in cls._fields_:
+            if ctype is C.c_void_p:
+                a = np.ascontiguousarray(g[name], dtype=np.float64)
+                keep[name] = a
+                setattr(s, name, a.ctypes.data)
+            else:
+                setattr(s, name, g[name])
+        s._keep = keep  # host arrays must outlive the struct
+        return s
+
+
+def _stream():
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+class AdzContext:
+    """adz_set + the device state of module adz_mem for one tile."""
+
+    def __init__(self, grid, pos_dtype=L.ADZ_F32, itraj=3):
+        self.grid = grid
+        self.pos_dtype = pos_dtype
+        self._g = AdzGrid.from_dict(grid)
+        h = C.c_void_p()
+        L.check(L.lib().adz_set(C.byref(self._g), pos_dtype, itraj, C.byref(h)), "adz_set")
+        self.h = h
+        iv = np.zeros(24, np.int32)
+        dv = np.zeros(9)
+        L.check(L.lib().adz_ctx_info(self.h, iv.ctypes.data, dv.ctypes.data), "adz_ctx_info")
+        self.info = dict(zip(INFO_INTS, iv.tolist()))
+        self.info.update(zip(INFO_DBLS, dv.tolist()))
+
+    def table(self, which):
+        nk = self.grid["nk"]
+        if which in (0, 1):
+            out = np.zeros(self.info["pnz_m" if which == 0 else "pnz_t"], np.int32)
+        elif which == 10:
+            out = np.zeros(self.info["njt"])
+        else:
+            out = np.zeros(nk + 1)
+        L.check(L.lib().adz_ctx_table(self.h, which, out.ctypes.data), "adz_ctx_table")
+        return out
+
+    def array(self, which):
+        """Device state array as a torch tensor view (no copy)."""
+        p = C.c_void_p()
+        n = C.c_long()
+        L.check(L.lib().adz_ctx_array(self.h, which, C.byref(p), C.byref(n)), "adz_ctx_array")
+        dt = torch.float64 if (which == ADZ_A_WPXYZ or self.pos_dtype == L.ADZ_F64) else torch.float32
+        return _wrap(p.value, n.value, dt)
+
+    def ztab(self, thermo):
+        return L.lib().adz_ctx_ztab(self.h, 1 if thermo else 0)
+
+    def adz_inittraj(self):
+        L.check(L.lib().adz_inittraj(self.h, _stream()), "adz_inittraj")
+
+    def adz_traject(self, dt, uu_ext, vv_ext, ww_ext, ut0, vt0, zdt0, schm_advec=1, bA_m=0.6, bA=0.6):
+        args = [uu_ext, vv_ext, ww_ext, ut0, vt0, zdt0]
+        for a in args:
+            assert a.is_cuda and a.dtype == torch.float32 and a.is_contiguous()
+        L.check(L.lib().adz_traject(self.h, float(dt), int(schm_advec), float(bA_m), float(bA),
+                                    *[C.c_void_p(a.data_ptr()) for a in args], _stream()), "adz_traject")
+
+    def free(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            L.lib().adz_ctx_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class _DevBuf:
+    """Minimal __cuda_array_interface__ provider to view library-owned device memory."""
+
+    def __init__(self, ptr, n, typestr):
+        self.__cuda_array_interface__ = {"shape": (n,), "typestr": typestr, "data": (ptr, False), "version": 2,
+                                         "strides": None}
+
+
+def _wrap(ptr, n, dt):
+    typestr = "<f8" if dt == torch.float64 else "<f4"
+    return torch.as_tensor(_DevBuf(ptr, n, typestr), device="cuda")

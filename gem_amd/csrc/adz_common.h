@@ -163,6 +163,40 @@ __device__ __forceinline__ void weights_f32(const Index& x, const ZtabDev& lv, f
   c[7] = s6 * py * qp1 * qm1;
 }
 
+// Three interleaved components f(3,ni,nj,nk) in reference arithmetic: z by mul + 3 fma,
+// rows folded by fma, and the x stage accumulated column by column with fma
+// (Tricublin_zyxf3_inline AVX2 path, turbo_tricublin.c:1142-1257).  f points at the
+// stencil corner; ni3/nij3 are the element strides of a row / plane (3*ni, 3*nij).
+template <class TF>
+__device__ __forceinline__ void tricub3_ref(const TF* __restrict__ f, const double* c, long ni3, long nij3,
+                                            int zlinear, double* out) {
+  const long o1 = zlinear ? 0 : nij3;
+  const long o2 = o1 + nij3, o3 = o2 + o1;
+  double dst[12];
+#pragma unroll
+  for (int e = 0; e < 12; e++) {
+    double ya = 0.0;
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const TF* s = f + r * ni3;
+      double za = to_f64(s[e]) * c[8];
+      za = fma(to_f64(s[e + o1]), c[9], za);
+      za = fma(to_f64(s[e + o2]), c[10], za);
+      za = fma(to_f64(s[e + o3]), c[11], za);
+      ya = (r == 0) ? za * c[4] : fma(za, c[4 + r], ya);
+    }
+    dst[e] = ya;
+  }
+#pragma unroll
+  for (int q = 0; q < 3; q++) {
+    double x0 = dst[q] * c[0];
+    x0 = fma(dst[q + 3], c[1], x0);
+    x0 = fma(dst[q + 6], c[2], x0);
+    x0 = fma(dst[q + 9], c[3], x0);
+    out[q] = x0;
+  }
+}
+
 template <class TP>
 __device__ __forceinline__ void load_pos(const void* pxyz, long p, double* x, double* y, double* z) {
   const TP* q = (const TP*)pxyz + 3 * p;

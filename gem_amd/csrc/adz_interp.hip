@@ -181,31 +181,11 @@ __global__ __launch_bounds__(256) void k_tricub3(TF* __restrict__ d, const TF* _
   const long o2 = o1 + nij, o3 = o2 + o1;
   const TF* f = f3 + 3 * x.off;
   if (MATH == ADZ_MATH_REF) {
-    double c[20];
+    double c[20], o[3];
     weights_ref(x, lv, c);
-    double dst[12];
+    tricub3_ref<TF>(f, c, ni, nij, x.zlinear, o);
 #pragma unroll
-    for (int e = 0; e < 12; e++) {
-      double ya = 0.0;
-#pragma unroll
-      for (int r = 0; r < 4; r++) {
-        const TF* s = f + r * ni;
-        double za = to_f64(s[e]) * c[8];
-        za = fma(to_f64(s[e + o1]), c[9], za);
-        za = fma(to_f64(s[e + o2]), c[10], za);
-        za = fma(to_f64(s[e + o3]), c[11], za);
-        ya = (r == 0) ? za * c[4] : fma(za, c[4 + r], ya);
-      }
-      dst[e] = ya;
-    }
-#pragma unroll
-    for (int q = 0; q < 3; q++) {
-      double x0 = dst[q] * c[0];
-      x0 = fma(dst[q + 3], c[1], x0);
-      x0 = fma(dst[q + 6], c[2], x0);
-      x0 = fma(dst[q + 9], c[3], x0);
-      d[3 * p + q] = from_f64<TF>(x0);
-    }
+    for (int q = 0; q < 3; q++) d[3 * p + q] = from_f64<TF>(o[q]);
   } else {
     float c[20];
     weights_f32(x, lv, c);
@@ -362,7 +342,7 @@ int adz_vsearch_setup_plus(const double* levels, int nk, int ni, int nj, int off
       hipMemcpy(docz, ocz.data(), 32 * nk, hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(dhz, hz.data(), 16 * nk, hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(doczf, oczf.data(), 16 * nk, hipMemcpyHostToDevice) != hipSuccess) {
-    hipFree(mem);
+    (void)hipFree(mem);
     return fail(ADZ_EHIP, "vsearch_setup: hipMemcpy failed");
   }
   adz_ztab* t = new adz_ztab;

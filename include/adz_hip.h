@@ -85,6 +85,81 @@ int adz_tricublin_mono_zyx_m_n(void* d, void* l, void* mi, void* ma, const void*
 int adz_tricublin_zyx3_n(void* d, const void* f3, const void* pxyz, const adz_ztab* lv, int n, int fdtype,
                          int pdtype, int math, adz_stream stream);
 
+
+/* ------------------------------------------------------------------------------------
+ * Level 2: the adz_* layer (replaces the Fortran routines of gemdyn/src/base/adz_*.F90
+ * on device-resident state).  One context per model tile.
+ * ----------------------------------------------------------------------------------*/
+
+/* What adz_set reads from GEM's modules (glb_ld, ver, geomh, dcst, lam/grid options).
+ * zm/zt/lat are HOST arrays: Ver_z_8%m(0:nk+1), Ver_z_8%t(0:nk+1), Ver_z_8%x(0:nk) and
+ * the latitude (radians) of local rows lminy..lmaxy. */
+typedef struct adz_grid {
+  int ni, nj, nk;                    /* l_ni, l_nj, l_nk (= G_nk)                        */
+  int i0g, j0g;                      /* l_i0, l_j0: global index of local (1,1)         */
+  int halox, haloy;                  /* Adz_halox / Adz_haloy (= Grd_maxcfl + 1)         */
+  int west, east, south, north;      /* tile touches the domain edge (1/0)               */
+  int pil_w, pil_e, pil_s, pil_n;    /* Glb_pil_* (used where the tile touches the edge) */
+  int gbpil_t;                       /* Lam_gbpil_T                                      */
+  int bc_lam_flux;                   /* Adz_BC_LAM_flux                                  */
+  int niu, njv;                      /* l_niu, l_njv                                     */
+  double inv_hx, inv_hy;             /* geomh_inv_hx_8, geomh_inv_hy_8                   */
+  double inv_rayt;                   /* Dcst_inv_rayt_8                                  */
+  const double* zm;
+  const double* zt;
+  const double* zx;
+  const double* lat;
+  double zmin, zmax;                 /* Ver_zmin_8, Ver_zmax_8                           */
+  double wmstar_nk;                  /* Ver_wmstar_8(nk)                                 */
+  double idz_t_nkm1;                 /* Ver_idz_8%t(nk-1)                                */
+} adz_grid;
+
+typedef struct adz_ctx adz_ctx;
+
+/* adz_set (adz_set.F90:16-311, single tile, ADZ_OD_L = .false.): bounds, clamps, level
+ * search tables, vertical Lagrange weights, cos-latitude factors, the two vertical
+ * descriptors and the device state.  pos_dtype selects the storage of the derived winds
+ * and departure positions: ADZ_F32 = the reference's declared `real` arrays
+ * (adz_mem.F90:78-84), bit-exact trajectories; ADZ_F64 = FP64 throughout.  itraj is
+ * Adz_itraj (adz_options.F90:30, default 3). */
+int adz_set(const adz_grid* g, int pos_dtype, int itraj, adz_ctx** out);
+int adz_ctx_free(adz_ctx* c);
+/* Derived integers (24): lminx lmaxx lminy lmaxy nit njt nij i0 in j0 jn i0u inu j0v jnv
+ * i0b inb j0b jnb k0 k0t k0m pnz_m pnz_t; doubles (9): iminposx imaxposx iminposy
+ * imaxposy sig ovdzm ovdzt vw5 vw6.  No reference counterpart (module variables there). */
+int adz_ctx_info(const adz_ctx* c, int* ints, double* dbls);
+/* Copy a derived table to the host: 0 search_m, 1 search_t (int); 2..9 adz_vw1m..4m,
+ * adz_vw1t..4t (nk+1 doubles, index k); 10 cy (njt); 11 odelz_m; 12 odelz_t. */
+int adz_ctx_table(const adz_ctx* c, int which, void* host_out);
+
+/* State arrays owned by the context (device pointers): */
+#define ADZ_A_PXYZM 0   /* Adz_pxyzm(3, ni, nj, k0m:nk)           pos_dtype */
+#define ADZ_A_WPXYZ 1   /* Adz_wpxyz(-1:ni+2, -1:nj+2, nk, 3)      FP64      */
+#define ADZ_A_UVW_DEP 2 /* Adz_uvw_dep(3, ni, nj, k0m:nk)          pos_dtype */
+#define ADZ_A_PM 3      /* Adz_pm (3, i0:in,   j0:jn,   k0:nk)     pos_dtype */
+#define ADZ_A_PMU 4     /* Adz_pmu(3, i0u:inu, j0:jn,   k0:nk)     pos_dtype */
+#define ADZ_A_PMV 5     /* Adz_pmv(3, i0:in,   j0v:jnv, k0:nk)     pos_dtype */
+#define ADZ_A_PT 6      /* Adz_pt (3, i0:in,   j0:jn,   k0t:nk)    pos_dtype */
+#define ADZ_A_PB 7      /* Adz_pb (3, i0b:inb, j0b:jnb, 1:nk)      pos_dtype */
+#define ADZ_A_UVW_D 8   /* Adz_uvw_d(3, lminx:lmaxx, lminy:lmaxy, nk) pos_dtype */
+#define ADZ_A_WARR 9    /* Adz_uu_arr, Adz_vv_arr, Adz_ww_arr (ni, nj, nk) x3   */
+int adz_ctx_array(adz_ctx* c, int which, void** dptr, long* nelem);
+/* The vertical descriptors adz_set builds (Adz_cpntr_q / Adz_cpntr_t); owned by c. */
+const adz_ztab* adz_ctx_ztab(const adz_ctx* c, int thermo);
+
+/* adz_inittraj (adz_inittraj.F90:16-42): arrival points as first guess, and the next
+ * trajectory uses max(6, itraj) iterations. */
+int adz_inittraj(adz_ctx* c, adz_stream stream);
+
+/* adz_traject (adz_traject.F90:16-130) with adz_prepareWinds and adz_interp_traj: winds
+ * are FP32 DEVICE arrays on the extended layout (lminx:lmaxx, lminy:lmaxy, nk): uu_ext,
+ * vv_ext, ww_ext (Adz_uu/vv/ww_ext), ut0, vt0 (staggered winds at time 0, halo >= 2) and
+ * zdt0 (zeta-dot at time 0).  schm_advec, bA_m, bA as Schm_advec, Cstv_bA_m_8, Cstv_bA_8.
+ * Fills every position set of the context. */
+int adz_traject(adz_ctx* c, double dt, int schm_advec, double bA_m, double bA, const float* uu_ext,
+                const float* vv_ext, const float* ww_ext, const float* ut0, const float* vt0, const float* zdt0,
+                adz_stream stream);
+
 #ifdef __cplusplus
 }
 #endif

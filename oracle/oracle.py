@@ -201,3 +201,103 @@ class RefKernel:
         d = np.zeros(3 * n, np.float32)
         self.L.Tricublin_zyx3_n(_p(d), C.c_void_p(f3.ctypes.data + int(base_off) * 4), _p(pos), lv, n)
         return d.reshape(n, 3)
+
+
+# ------------------------------------------------------------------ adz_* layer
+class GridDesc(C.Structure):
+    """struct GridDesc of oracle_adz.h (same layout as the library's adz_grid)."""
+    _fields_ = [(n, C.c_int) for n in ("ni", "nj", "nk", "i0g", "j0g", "halox", "haloy", "west", "east", "south",
+                                       "north", "pil_w", "pil_e", "pil_s", "pil_n", "gbpil_t", "bc_lam_flux", "niu",
+                                       "njv")] + \
+              [("inv_hx", C.c_double), ("inv_hy", C.c_double), ("inv_rayt", C.c_double)] + \
+              [(n, C.c_void_p) for n in ("zm", "zt", "zx", "lat")] + \
+              [(n, C.c_double) for n in ("zmin", "zmax", "wmstar_nk", "idz_t_nkm1")]
+
+
+INFO_INTS = ["lminx", "lmaxx", "lminy", "lmaxy", "nit", "njt", "nij", "i0", "in", "j0", "jn", "i0u", "inu", "j0v",
+             "jnv", "i0b", "inb", "j0b", "jnb", "k0", "k0t", "k0m", "pnz_m", "pnz_t"]
+INFO_DBLS = ["iminposx", "imaxposx", "iminposy", "imaxposy", "sig", "ovdzm", "ovdzt", "vw5", "vw6"]
+
+
+class Grid:
+    """Oracle adz_set restatement for one tile."""
+
+    def __init__(self, g):
+        self.g = g
+        d = GridDesc()
+        self._keep = {}
+        for name, ctype in GridDesc._fields_:
+            if ctype is C.c_void_p:
+                a = np.ascontiguousarray(g[name], dtype=np.float64)
+                self._keep[name] = a
+                setattr(d, name, a.ctypes.data)
+            else:
+                setattr(d, name, g[name])
+        self._d = d
+        self.h = lib().ora_grid_new(C.byref(d))
+        iv = np.zeros(24, np.int32)
+        dv = np.zeros(9)
+        lib().ora_grid_info(self.h, _p(iv), _p(dv))
+        self.info = dict(zip(INFO_INTS, iv.tolist()))
+        self.info.update(zip(INFO_DBLS, dv.tolist()))
+
+    def table(self, which):
+        nk = self.g["nk"]
+        if which in (0, 1):
+            out = np.zeros(self.info["pnz_m" if which == 0 else "pnz_t"], np.int32)
+        elif which == 10:
+            out = np.zeros(self.info["njt"])
+        else:
+            out = np.zeros(nk + 1)
+        lib().ora_grid_table(self.h, which, _p(out))
+        return out
+
+    def level_search(self, which, pos):
+        pos = np.ascontiguousarray(pos, dtype=np.float64)
+        kk = np.zeros(pos.size, np.int32)
+        lib().ora_level_search(self.h, which, pos.size, _p(pos), _p(kk))
+        return kk
+
+    def zindx(self, pos):
+        pos = np.ascontiguousarray(pos, dtype=np.float64)
+        out = np.zeros(pos.size)
+        lib().ora_zindx(self.h, pos.size, _p(pos), _p(out))
+        return out
+
+    def sizes(self):
+        i, g = self.info, self.g
+        ni, nj, nk = g["ni"], g["nj"], g["nk"]
+        nxy = ni * nj
+        return dict(p3=3 * nxy * (nk - i["k0m"] + 1), wp=(ni + 4) * (nj + 4) * nk * 3,
+                    pm=3 * (i["in"] - i["i0"] + 1) * (i["jn"] - i["j0"] + 1) * (nk - i["k0"] + 1),
+                    pmu=3 * (i["inu"] - i["i0u"] + 1) * (i["jn"] - i["j0"] + 1) * (nk - i["k0"] + 1),
+                    pmv=3 * (i["in"] - i["i0"] + 1) * (i["jnv"] - i["j0v"] + 1) * (nk - i["k0"] + 1),
+                    pt=3 * (i["in"] - i["i0"] + 1) * (i["jn"] - i["j0"] + 1) * (nk - i["k0t"] + 1),
+                    pb=3 * (i["inb"] - i["i0b"] + 1) * (i["jnb"] - i["j0b"] + 1) * nk,
+                    uvw_d=3 * i["nij"] * nk, arr=3 * nxy * nk)
+
+    def traject(self, winds, dt, storage="f32", state=None, niter=3, advec=1, bA_m=0.6, bA=0.6, order=ORD_AVX2):
+        """adz_traject restatement.  storage 'f32' = reference `real` storage, 'f64' = FP64.
+        state: dict with 'pxyzm' and 'wpxyz' carried from a previous step (None: inittraj,
+        and then max(6, niter) iterations as adz_inittraj sets)."""
+        dt_ = np.float32 if storage == "f32" else np.float64
+        sz = self.sizes()
+        out = {k: np.zeros(sz[s], dt_) for k, s in (("uvw_dep", "p3"), ("pm", "pm"), ("pmu", "pmu"), ("pmv", "pmv"),
+                                                    ("pt", "pt"), ("pb", "pb"), ("uvw_d", "uvw_d"), ("warr", "arr"))}
+        init = state is None
+        pxyzm = np.zeros(sz["p3"], dt_) if init else np.ascontiguousarray(state["pxyzm"], dtype=dt_).copy()
+        wpxyz = np.zeros(sz["wp"]) if init else np.ascontiguousarray(state["wpxyz"], dtype=np.float64).copy()
+        nit = max(6, niter) if init else niter
+        w = {k: np.ascontiguousarray(np.asarray(winds[k]), dtype=np.float32)
+             for k in ("uu_ext", "vv_ext", "ww_ext", "ut0", "vt0", "zdt0")}
+        lib().ora_traject(self.h, 1 if storage == "f32" else 0, 1 if init else 0, dt, advec, bA_m, bA, nit, order,
+                          *[_p(w[k]) for k in ("uu_ext", "vv_ext", "ww_ext", "ut0", "vt0", "zdt0")],
+                          _p(pxyzm), _p(wpxyz), *[_p(out[k]) for k in ("uvw_dep", "pm", "pmu", "pmv", "pt", "pb",
+                                                                       "uvw_d", "warr")])
+        out["pxyzm"], out["wpxyz"] = pxyzm, wpxyz
+        return out
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.ora_grid_free(self.h)
+            self.h = None

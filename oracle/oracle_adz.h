@@ -469,10 +469,14 @@ inline void traject(const Grid& g, double dt, int schm_advec, double bA_m, doubl
   prepare_winds<S>(g, uu_ext, vv_ext, ww_ext, ut0, vt0, zdt0, s);
   int k00 = g.k0m;
   if (g.k0 > 1) k00 = 1;
-  for (int it = 1; it <= niter; it++)
+  // each arrival point only reads the static winds and its own iterate, so the points of
+  // one iteration are independent (the reference shares them out per level, :67-68)
+  for (int it = 1; it <= niter; it++) {
+#pragma omp parallel for collapse(2) schedule(static)
     for (int k = k00; k <= g.nk; k++)
       for (int j = 1; j <= g.nj; j++)
         for (int i = 1; i <= g.ni; i++) traj_point<S>(g, t, i, j, k, order, s);
+  }
   for (int k = g.k0; k <= g.nk; k++)
     for (int j = g.j0; j <= g.jn; j++)
       for (int i = g.i0; i <= g.in; i++) {
