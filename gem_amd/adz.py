@@ -130,3 +130,113 @@ class _DevBuf:
 def _wrap(ptr, n, dt):
     typestr = "<f8" if dt == torch.float64 else "<f4"
     return torch.as_tensor(_DevBuf(ptr, n, typestr), device="cuda")
+
+
+# ---------------------------------------------------------------------- adz_tricub_hlt
+L.extra_sigs({
+    "adz_tricub_hlt": ("i", ["vp", "i", "vp", "i", "vp"] + ["i"] * 10 + ["i", "i", "i", "i", "vp"]),
+    "adz_plan_create": ("i", ["vp", "i", "i", "i", "i", "vp", "vp"]),
+    "adz_plan_execute": ("i", ["vp", "vp"]),
+    "adz_plan_tune": ("i", ["vp", "vp"]),
+    "adz_plan_free": ("i", ["vp"]),
+})
+
+ADZ_POST_STORE, ADZ_POST_CLIP = 1, 2
+_DT = {torch.float64: L.ADZ_F64, torch.float32: L.ADZ_F32, torch.bfloat16: L.ADZ_BF16}
+
+
+class AdzField(C.Structure):
+    _fields_ = [("src", C.c_void_p), ("dst", C.c_void_p), ("lin", C.c_void_p), ("mn", C.c_void_p),
+                ("mx", C.c_void_p)]
+
+
+class AdzItem(C.Structure):
+    _fields_ = [("geom", C.c_void_p), ("xyz", C.c_void_p), ("pdtype", C.c_int)] + \
+               [(n, C.c_int) for n in ("i0", "in_", "j0", "jn", "k0", "nk", "dminx", "dmaxx", "dminy", "dmaxy",
+                                       "ext")] + [("fld", C.c_void_p), ("nptr", C.c_int)]
+
+
+class AdzTune(C.Structure):
+    _fields_ = [("fields_per_pass", C.c_int), ("points_outer", C.c_int), ("blocks_per_cu", C.c_int),
+                ("xcd_chunks", C.c_int)]
+
+
+def _ptr_at(t, off):
+    return None if t is None else t.data_ptr() + int(off) * t.element_size()
+
+
+def field_list(stack, src_off, dst_off):
+    """stack: list of dicts with tensors 'src', 'dst' and optional 'lin','min','max'
+    (the Adz_pntr_stack / Adz_post entries); offsets address element (1,1,1)."""
+    arr = (AdzField * len(stack))()
+    for n, e in enumerate(stack):
+        arr[n].src = _ptr_at(e["src"], src_off)
+        arr[n].dst = _ptr_at(e["dst"], dst_off)
+        for k, a in (("lin", "lin"), ("mn", "min"), ("mx", "max")):
+            setattr(arr[n], k, _ptr_at(e.get(a), dst_off))
+    return arr
+
+
+def adz_tricub_hlt(stack, xyz, geom, i0, in_, j0, jn, k0, nk, dlayout, ext=True, post=0, math=L.ADZ_MATH_REF,
+                   src_off=None):
+    """adz_tricub_hlt(F_stk, F_nptr, F_xyz, F_geom, F_num, F_i0, F_in, F_j0, F_jn, F_k0
+    [,F_ext_L, F_post]).  dlayout = (dminx, dmaxx, dminy, dmaxy) of the destination (model)
+    arrays; geom = an adz_ztab handle (e.g. AdzContext.ztab(thermo)); src_off = element
+    offset of src(1,1,1) (default: the extended layout of the context)."""
+    dminx, dmaxx, dminy, dmaxy = dlayout
+    dst_off = (1 - dminx) + (dmaxx - dminx + 1) * (1 - dminy)
+    if src_off is None:
+        src_off = dst_off
+    fl = field_list(stack, src_off, dst_off)
+    fdt = _DT[stack[0]["dst"].dtype]
+    pdt = _DT[xyz.dtype]
+    L.check(L.lib().adz_tricub_hlt(fl, len(stack), C.c_void_p(xyz.data_ptr()), pdt, geom, i0, in_, j0, jn, k0, nk,
+                                   dminx, dmaxx, dminy, dmaxy, 1 if ext else 0, post, fdt, math, _stream()),
+            "adz_tricub_hlt")
+
+
+class Plan:
+    """A persistent batched launch over several subdomains (adz_plan_*)."""
+
+    def __init__(self, items, fdtype, math=L.ADZ_MATH_F32, post=0, tune=None):
+        self._keep = []
+        arr = (AdzItem * len(items))()
+        for q, it in enumerate(items):
+            fl = it["fields"]
+            self._keep.append(fl)
+            a = arr[q]
+            a.geom = it["geom"]
+            a.xyz = it["xyz"].data_ptr()
+            a.pdtype = _DT[it["xyz"].dtype]
+            a.i0, a.in_, a.j0, a.jn, a.k0, a.nk = it["box"]
+            a.dminx, a.dmaxx, a.dminy, a.dmaxy = it["dlayout"]
+            a.ext = 1 if it.get("ext", True) else 0
+            a.fld = C.cast(fl, C.c_void_p)
+            a.nptr = len(fl)
+        t = None
+        if tune is not None:
+            t = AdzTune(*[int(tune.get(k, 0)) for k in ("fields_per_pass", "points_outer", "blocks_per_cu",
+                                                         "xcd_chunks")])
+        h = C.c_void_p()
+        L.check(L.lib().adz_plan_create(arr, len(items), fdtype, math, post, C.byref(t) if t else None,
+                                        C.byref(h)), "adz_plan_create")
+        self.h = h
+
+    def execute(self):
+        L.check(L.lib().adz_plan_execute(self.h, _stream()), "adz_plan_execute")
+
+    def tune(self):
+        t = AdzTune()
+        L.check(L.lib().adz_plan_tune(self.h, C.byref(t)), "adz_plan_tune")
+        return {k: getattr(t, k) for k, _ in AdzTune._fields_}
+
+    def free(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            L.lib().adz_plan_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass

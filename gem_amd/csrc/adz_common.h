@@ -197,6 +197,16 @@ __device__ __forceinline__ void tricub3_ref(const TF* __restrict__ f, const doub
   }
 }
 
+}  // namespace adz
+
+// Host handle of a vertical descriptor (shared by every translation unit).
+struct adz_ztab {
+  adz::ZtabDev dev;
+  void* mem;
+};
+
+namespace adz {
+
 template <class TP>
 __device__ __forceinline__ void load_pos(const void* pxyz, long p, double* x, double* y, double* z) {
   const TP* q = (const TP*)pxyz + 3 * p;

@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "adz_common.h"
+#include "adz_eval.h"
 
 namespace adz {
 
@@ -26,86 +27,6 @@ constexpr int kMaxFieldsPerLaunch = 64;
 struct FieldPtrs {
   const void* f[kMaxFieldsPerLaunch];
 };
-
-// ------------------------------------------------------------------ field evaluators
-// Reference arithmetic, AVX2 order (turbo_tricublin.c:553-615): per row, z by one mul and
-// three fma, rows folded into y by fma, x by (y0c0+y2c2)+(y1c1+y3c3).
-template <class TF>
-__device__ __forceinline__ double tricub_ref(const TF* __restrict__ f1, const double* c, long ni, long o1, long o2,
-                                             long o3) {
-  double ya[4];
-#pragma unroll
-  for (int r = 0; r < 4; r++) {
-    const TF* s = f1 + r * ni;
-#pragma unroll
-    for (int i = 0; i < 4; i++) {
-      double za = to_f64(s[i]) * c[8];
-      za = fma(to_f64(s[i + o1]), c[9], za);
-      za = fma(to_f64(s[i + o2]), c[10], za);
-      za = fma(to_f64(s[i + o3]), c[11], za);
-      ya[i] = (r == 0) ? za * c[4] : fma(za, c[4 + r], ya[i]);
-    }
-  }
-  double y0 = ya[0] * c[0], y1 = ya[1] * c[1], y2 = ya[2] * c[2], y3 = ya[3] * c[3];
-  return (y0 + y2) + (y1 + y3);
-}
-
-// FP32 arithmetic of the mixed form, same contraction structure.
-template <class TF>
-__device__ __forceinline__ float tricub_f32(const TF* __restrict__ f1, const float* c, long ni, long o1, long o2,
-                                            long o3) {
-  float ya[4];
-#pragma unroll
-  for (int r = 0; r < 4; r++) {
-    const TF* s = f1 + r * ni;
-#pragma unroll
-    for (int i = 0; i < 4; i++) {
-      float za = to_f32(s[i]) * c[8];
-      za = fmaf(to_f32(s[i + o1]), c[9], za);
-      za = fmaf(to_f32(s[i + o2]), c[10], za);
-      za = fmaf(to_f32(s[i + o3]), c[11], za);
-      ya[i] = (r == 0) ? za * c[4] : fmaf(za, c[4 + r], ya[i]);
-    }
-  }
-  return fmaf(ya[3], c[3], fmaf(ya[2], c[2], fmaf(ya[1], c[1], ya[0] * c[0])));
-}
-
-// min / max / linear of the monotone interpolator (turbo_tricublin.c:883-989): min/max
-// over x columns 1..2, y rows 0..3, z planes 1..2 with the MINPD/MAXPD operand rule; the
-// linear result from rows 1..2 and planes 1..2.
-template <class TF, class TC>
-__device__ __forceinline__ void mono_extra(const TF* __restrict__ f1, const TC* c, long ni, long o1, long o2, TC* lin,
-                                           TC* mi, TC* ma) {
-  TC vmi[4], vma[4];
-#pragma unroll
-  for (int r = 0; r < 4; r++) {
-    const TF* s = f1 + r * ni;
-#pragma unroll
-    for (int i = 1; i < 3; i++) {
-      TC a = (TC)to_f64(s[i + o1]), b = (TC)to_f64(s[i + o2]);
-      if (r == 0) { vmi[i] = a; vma[i] = a; }
-      else { vmi[i] = (vmi[i] < a) ? vmi[i] : a; vma[i] = (vma[i] > a) ? vma[i] : a; }
-      vmi[i] = (vmi[i] < b) ? vmi[i] : b;
-      vma[i] = (vma[i] > b) ? vma[i] : b;
-    }
-  }
-  *mi = (vmi[1] < vmi[2]) ? vmi[1] : vmi[2];
-  *ma = (vma[1] > vma[2]) ? vma[1] : vma[2];
-  TC vyl[4];
-#pragma unroll
-  for (int i = 0; i < 4; i++) {
-    const TF* s1 = f1 + ni;
-    const TF* s2 = f1 + 2 * ni;
-    TC vz1 = (TC)to_f64(s1[i + o1]) * c[18];
-    vz1 = fma(((TC)to_f64(s1[i + o2])), c[19], vz1);
-    TC vy = vz1 * c[16];
-    TC vz2 = (TC)to_f64(s2[i + o1]) * c[18];
-    vz2 = fma((TC)to_f64(s2[i + o2]), c[19], vz2);
-    vyl[i] = fma(vz2, c[17], vy);
-  }
-  TC v0 = vyl[0] * c[12], v1 = vyl[1] * c[13], v2 = vyl[2] * c[14], v3 = vyl[3] * c[15];
-  *lin = (v0 + v2) + (v1 + v3);
-}
 
 // ------------------------------------------------------------------ kernels
 enum { K_ZYX1 = 0, K_MONO = 1 };
@@ -241,11 +162,6 @@ __global__ __launch_bounds__(256) void k_vcoef(int* __restrict__ ijk, int* __res
 
 // ====================================================================== C ABI
 using namespace adz;
-
-struct adz_ztab {
-  ZtabDev dev;
-  void* mem;
-};
 
 namespace {
 inline unsigned nblocks(long n) { return (unsigned)((n + 255) / 256); }

@@ -309,11 +309,6 @@ __global__ void k_copy_box(S* __restrict__ out, const S* __restrict__ src, TrajP
 }  // namespace adz
 
 // ====================================================================== host side
-struct adz_ztab {
-  ZtabDev dev;
-  void* mem;
-};
-
 struct adz_ctx {
   int pos_dtype, itraj, niter;
   TrajParams P;

@@ -160,6 +160,66 @@ int adz_traject(adz_ctx* c, double dt, int schm_advec, double bA_m, double bA, c
                 const float* vv_ext, const float* ww_ext, const float* ut0, const float* vt0, const float* zdt0,
                 adz_stream stream);
 
+/* ------------------------------------------------------------------------------------
+ * Level 3: fused multi-field interpolation (adz_tricub_hlt) and the batched, persistent
+ * form over several subdomains.
+ * ----------------------------------------------------------------------------------*/
+
+/* One field of a group (the reference's Adz_pntr_stack entry, adz_mem.F90:66-72, plus the
+ * Adz_post arrays of adz_mem.F90:92-98).  All are DEVICE pointers to element (1,1,1). */
+typedef struct adz_field {
+  const void* src; /* source; halo-extended layout of `geom` when ext != 0, else the
+                      model layout (dminx:dmaxx, dminy:dmaxy, nk)                     */
+  void* dst;       /* destination, model layout                                       */
+  void* lin;       /* Adz_post(n)%lin (clipped to [min,max]); with ADZ_POST_STORE      */
+  void* mn;        /* Adz_post(n)%min                                                 */
+  void* mx;        /* Adz_post(n)%max                                                 */
+} adz_field;
+
+#define ADZ_POST_STORE 1 /* store lin/min/max (adz_interp_hlt.F90:146-163)              */
+#define ADZ_POST_CLIP 2  /* clip dst to [min, max] (mono=1, adz_post_tr.F90:81-94)      */
+
+/* adz_tricub_hlt (adz_interp_hlt.F90:28-187): interpolate nptr fields at the positions
+ * xyz of the target box (i0:in, j0:jn, k0:nk) (i fastest, the layout of Adz_pm/pmu/pmv/pt/
+ * pb) and write each result into dst(i,j,k).  `fld` is a HOST array of nptr entries.
+ * ext = 0 interpolates straight from the model array (no copy into an extended work
+ * array: on one tile every stencil lies inside it; a stencil that does not yields NaN). */
+int adz_tricub_hlt(const adz_field* fld, int nptr, const void* xyz, int pdtype, const adz_ztab* geom, int i0,
+                   int in, int j0, int jn, int k0, int nk, int dminx, int dmaxx, int dminy, int dmaxy, int ext,
+                   int post, int fdtype, int math, adz_stream stream);
+
+/* A batched work item: one subdomain (or position set) with its own descriptor,
+ * positions and field group (the arguments of one adz_tricub_hlt call). */
+typedef struct adz_item {
+  const adz_ztab* geom;
+  const void* xyz;
+  int pdtype;
+  int i0, in, j0, jn, k0, nk;
+  int dminx, dmaxx, dminy, dmaxy;
+  int ext;
+  const adz_field* fld; /* HOST array of nptr fields (copied into the plan) */
+  int nptr;
+} adz_item;
+
+/* Tile parameters of the persistent kernel; zero fields take the autotuned default. */
+typedef struct adz_tune {
+  int fields_per_pass; /* fields interpolated per pass over the points (1..16)          */
+  int points_outer;    /* 1: a block runs every field group of a point tile; 0: a pass
+                          over all points per field group                               */
+  int blocks_per_cu;   /* persistent grid = blocks_per_cu * CUs                         */
+  int xcd_chunks;      /* 1: each XCD takes a contiguous range of tiles                 */
+} adz_tune;
+
+typedef struct adz_plan adz_plan;
+/* Build a plan for nitems subdomains: uploads the descriptors once, so that executing
+ * it launches ONE persistent kernel with no host work (capturable in a hipGraph).
+ * No reference counterpart (the reference loops over slices on OpenMP threads). */
+int adz_plan_create(const adz_item* items, int nitems, int fdtype, int math, int post, const adz_tune* tune,
+                    adz_plan** out);
+int adz_plan_execute(const adz_plan* p, adz_stream stream);
+int adz_plan_tune(const adz_plan* p, adz_tune* out); /* the parameters in use */
+int adz_plan_free(adz_plan* p);
+
 #ifdef __cplusplus
 }
 #endif

@@ -326,7 +326,7 @@ template <class S>
 inline void traj_point(const Grid& g, const Dt& t, int i, int j, int k, int order, TrajState<S>* s) {
   long p = p3_idx(g.ni, g.nj, g.k0m, i, j, k);
   double c[24]; int off;
-  int zl = vcoef<S>(c, &off, s->pxyzm[p], s->pxyzm[p + 1], s->pxyzm[p + 2], g.ztab_q, 0);
+  int zl = vcoef<S>(c, &off, s->pxyzm[p], s->pxyzm[p + 1], s->pxyzm[p + 2], g.ztab_q, order == ORD_AVX2 ? 1 : 0);  // -mfma build fuses the absolute z
   // f3 passed as Adz_uvw_d(1,1,1,1): offset of local (1,1,1) in the extended array
   long base = 3L * ((long)(1 - g.lminx) + (long)g.nit * (1 - g.lminy));
   double w3[3];
