@@ -240,3 +240,72 @@ class Plan:
             self.free()
         except Exception:
             pass
+
+
+# ---------------------------------------------------------------------- Bermejo-Conde
+L.extra_sigs({
+    "adz_bc_create": ("i", ["vp", "i", "vp", "i", "vp"]),
+    "adz_bermejo_conde": ("i", ["vp"] * 7 + ["vp"]),
+    "adz_bc_free": ("i", ["vp"]),
+})
+
+
+class AdzBcTracer(C.Structure):
+    _fields_ = [("p", C.c_void_p), ("m", C.c_void_p), ("lin", C.c_void_p), ("mn", C.c_void_p), ("mx", C.c_void_p),
+                ("legacy", C.c_int), ("weight", C.c_int), ("pexp_n", C.c_int)]
+
+
+class AdzBcDomain(C.Structure):
+    _fields_ = [(n, C.c_int) for n in ("minx", "maxx", "miny", "maxy", "nk", "i0", "in_", "j0", "jn", "k0",
+                                       "min_max")] + [("pexp_list", C.c_float * 9)]
+
+
+def bc_settings(mass):
+    """Tracer `mass=` code to (legacy, weight, pexp_n), adz_post_tr.F90:190-210."""
+    if mass == 1:
+        return 1, 1, 0
+    weight = (mass - 100) // 10
+    return 0, weight, mass - (100 + weight * 10)
+
+
+class BermejoConde:
+    """Bermejo_Conde(F_bc, F_ntr_bc, F_i0, F_in, F_j0, F_jn, F_k0) on device arrays.
+
+    tracers: list of dicts with tensors p, m, lin, min, max (model layout, element (1,1,1)
+    at offset `off`) and the tracer's `mass` code (1 or 100+10*weight+pexp_n)."""
+
+    def __init__(self, tracers, layout, scope, off, min_max=True, pexp_list=(1.0,) * 9):
+        minx, maxx, miny, maxy, nk = layout
+        i0, in_, j0, jn, k0 = scope
+        arr = (AdzBcTracer * len(tracers))()
+        tdt = None
+        for n, t in enumerate(tracers):
+            legacy, weight, pexp_n = bc_settings(t["mass"])
+            a = arr[n]
+            a.p, a.m = _ptr_at(t["p"], off), _ptr_at(t["m"], off)
+            a.lin, a.mn, a.mx = _ptr_at(t["lin"], off), _ptr_at(t["min"], off), _ptr_at(t["max"], off)
+            a.legacy, a.weight, a.pexp_n = legacy, weight, pexp_n
+            tdt = _DT[t["m"].dtype]
+        dom = AdzBcDomain(minx, maxx, miny, maxy, nk, i0, in_, j0, jn, k0, 1 if min_max else 0,
+                          (C.c_float * 9)(*pexp_list))
+        h = C.c_void_p()
+        L.check(L.lib().adz_bc_create(arr, len(tracers), C.byref(dom), tdt, C.byref(h)), "adz_bc_create")
+        self.h, self.off, self.ntr = h, off, len(tracers)
+        self.mask_off = (1 - minx) + (maxx - minx + 1) * (1 - miny)
+
+    def __call__(self, airm1, airm0, mask, deficit, pkps=None, diag=None):
+        L.check(L.lib().adz_bermejo_conde(self.h, _ptr_at(airm1, self.off), _ptr_at(airm0, self.off),
+                                          _ptr_at(pkps, self.off), _ptr_at(mask, self.mask_off),
+                                          deficit.data_ptr(), None if diag is None else diag.data_ptr(),
+                                          _stream()), "adz_bermejo_conde")
+
+    def free(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            L.lib().adz_bc_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass

@@ -1,0 +1,330 @@
+// Bermejo-Conde mass fixer on gfx950 (src/gemdyn/src/base/Bermejo_Conde.F90:18-320, with
+// the global FP64 sums of mass_tr_PM.F90:75-175 and mass_tr_X.F90:66-111).
+//
+// The fixer needs four global sums per tracer, each one gating the next elementwise pass:
+//   1. masses of p (time P) and m (time M)        -> deficit D = M_m - (M_p - eps)
+//   2. mass of the weights w(m - lin, sign D)     -> lambda = D / W (skip if W is ~0)
+//   3. corrected m (min/max reset) and its mass  -> proportion = M_target / M_corrected
+//   4. proportional fix of m and its mass        -> new carried deficit eps
+// Every pass runs all tracers in one launch (grid.y = tracer).  Sums are deterministic:
+// each lane accumulates a fixed set of elements in double-double (TwoSum), blocks reduce
+// in a fixed tree, and a finalising kernel adds the block partials in index order; the
+// result is the exactly rounded sum except in astronomically rare ties, and independent of
+// how the domain is split.  The weights are recomputed in passes 2 and 3 instead of being
+// stored (w depends only on m, lin, pkps and the sign of D), which saves one array per
+// tracer.  Scalars stay on the device, so a timestep can be captured in a hipGraph.
+#include <cstring>
+#include <vector>
+
+#include "adz_common.h"
+
+namespace adz {
+
+constexpr int kBcBlock = 256;
+constexpr int kSc = 16;  // per-tracer scalars
+enum { SC_MP = 0, SC_MM, SC_MTP, SC_D, SC_W, SC_LAMBDA, SC_CYCLE, SC_MBCR, SC_PROP, SC_MBC };
+enum { P_PM = 0, P_W = 1, P_CORR = 2, P_PROP = 3 };
+
+struct BcTr {
+  const void* p;
+  void* m;
+  const void* lin;
+  const void* mn;
+  const void* mx;
+  int legacy, weight;
+  double pexp;
+};
+
+struct BcArgs {
+  const BcTr* tr;
+  int ntr;
+  int nxa, nya;              // array extents
+  int i0, j0, k0, nx, ny;    // scope
+  long npts;
+  int min_max;
+  int nblk;                  // blocks per tracer
+  long chunk;                // elements per block
+  const void* airm1;
+  const void* airm0;
+  const void* pkps;
+  const double* mask;
+  double* deficit;
+  double* diag;
+  double* part;              // [ntr][nblk][4]
+  double* sc;                // [ntr][kSc]
+};
+
+struct DD {
+  double hi, lo;
+};
+__device__ __forceinline__ void two_sum(double a, double b, double* s, double* e) {
+  double x = a + b;
+  double bp = x - a;
+  *e = (a - (x - bp)) + (b - bp);
+  *s = x;
+}
+__device__ __forceinline__ void dd_add(DD* a, double t) {
+  double s, e;
+  two_sum(a->hi, t, &s, &e);
+  a->hi = s;
+  a->lo += e;
+}
+__device__ __forceinline__ DD dd_merge(DD a, DD b) {
+  double s, e;
+  two_sum(a.hi, b.hi, &s, &e);
+  e += a.lo + b.lo;
+  DD r;
+  two_sum(s, e, &r.hi, &r.lo);
+  return r;
+}
+
+// almost_zero.F90:16-36 (see the oracle's note: true for the patterns 0 and 1 only)
+__device__ __forceinline__ bool d_almost_zero(double a) {
+  unsigned long long u = __builtin_bit_cast(unsigned long long, a);
+  long long abit = (long long)u;
+  if (abit < 0) abit = (long long)(0x80000000ULL - u);
+  return abit >= -1 && abit <= 1;
+}
+
+__device__ __forceinline__ double sgn1(double b) { return signbit(b) ? -1.0 : 1.0; }
+
+// w of Bermejo_Conde.F90:112-172 for one element (exactly the stored `real` weight)
+template <class T>
+__device__ __forceinline__ T bc_weight(T m, T lin, double pkps, const BcTr& t, double sd) {
+  T hml = m - lin;
+  double H = (double)hml;
+  T w = (T)0;
+  if (sd == sgn1(H)) {
+    double a = fabs(H);
+    double v = (t.pexp == 1.0) ? a : (t.pexp == 2.0 ? a * a : pow(a, t.pexp));
+    v = v * sd;
+    if (t.weight == 2) v = v * (double)(m > (T)0 ? m : (T)0);
+    if (t.weight == 3) v = v * pkps;
+    w = (T)v;
+  }
+  return w;
+}
+
+template <class T>
+__device__ __forceinline__ double mass_term(T x, T air, double mask) {
+  T prod = x * air;  // real*real is a `real` product in the Fortran sums
+  return (double)prod * mask;
+}
+
+template <class T, int PASS>
+__global__ __launch_bounds__(kBcBlock) void k_bc_pass(BcArgs A) {
+  const int n = blockIdx.y;
+  const BcTr t = A.tr[n];
+  const double* sc = A.sc + n * kSc;
+  const bool cyc = (PASS >= P_CORR) && sc[SC_CYCLE] != 0.0;
+  const double sd = sgn1(sc[SC_D]);
+  const double lambda = sc[SC_LAMBDA], prop = sc[SC_PROP];
+  DD s0{0.0, 0.0}, s1{0.0, 0.0};
+  long e0 = (long)blockIdx.x * A.chunk, e1 = e0 + A.chunk;
+  if (e1 > A.npts) e1 = A.npts;
+  const T* air1 = (const T*)A.airm1;
+  const T* air0 = (const T*)A.airm0;
+  for (long e = e0 + threadIdx.x; e < e1; e += kBcBlock) {
+    int i = A.i0 + (int)(e % A.nx);
+    long r = e / A.nx;
+    int j = A.j0 + (int)(r % A.ny);
+    int k = A.k0 + (int)(r / A.ny);
+    long q2 = (long)(i - 1) + (long)A.nxa * (j - 1);
+    long q = q2 + (long)A.nxa * A.nya * (k - 1);
+    double mk = A.mask[q2];
+    T* mp = (T*)t.m + q;
+    T m = *mp;
+    if (PASS == P_PM) {
+      dd_add(&s0, mass_term<T>(((const T*)t.p)[q], air1[q], mk));
+      dd_add(&s1, mass_term<T>(m, air0[q], mk));
+    } else if (PASS == P_W) {
+      double pk = (t.weight == 3) ? (double)((const T*)A.pkps)[q] : 0.0;
+      T w = bc_weight<T>(m, ((const T*)t.lin)[q], pk, t, sd);
+      dd_add(&s0, mass_term<T>(w, air0[q], mk));
+    } else if (PASS == P_CORR) {
+      if (!cyc) {
+        double pk = (t.weight == 3) ? (double)((const T*)A.pkps)[q] : 0.0;
+        T w = bc_weight<T>(m, ((const T*)t.lin)[q], pk, t, sd);
+        double corr = lambda * (double)w;
+        T bc = (T)((double)m - corr);
+        if (A.min_max) {
+          T mn = ((const T*)t.mn)[q], mx = ((const T*)t.mx)[q];
+          if (corr > 0.0 && bc < mn) bc = mn;
+          if (corr < 0.0 && bc > mx) bc = mx;
+        }
+        m = bc;
+        *mp = m;
+      }
+      dd_add(&s0, mass_term<T>(m, air0[q], mk));
+    } else {  // P_PROP
+      if (!cyc) {
+        m = (T)(prop * (double)m);
+        *mp = m;
+      }
+      dd_add(&s0, mass_term<T>(m, air0[q], mk));
+    }
+  }
+  // fixed-order block tree in LDS
+  __shared__ double sh[4][kBcBlock];
+  sh[0][threadIdx.x] = s0.hi; sh[1][threadIdx.x] = s0.lo;
+  sh[2][threadIdx.x] = s1.hi; sh[3][threadIdx.x] = s1.lo;
+  __syncthreads();
+  for (int w = kBcBlock / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) {
+      DD a{sh[0][threadIdx.x], sh[1][threadIdx.x]}, b{sh[0][threadIdx.x + w], sh[1][threadIdx.x + w]};
+      DD c = dd_merge(a, b);
+      sh[0][threadIdx.x] = c.hi; sh[1][threadIdx.x] = c.lo;
+      if (PASS == P_PM) {
+        DD a2{sh[2][threadIdx.x], sh[3][threadIdx.x]}, b2{sh[2][threadIdx.x + w], sh[3][threadIdx.x + w]};
+        DD c2 = dd_merge(a2, b2);
+        sh[2][threadIdx.x] = c2.hi; sh[3][threadIdx.x] = c2.lo;
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    double* o = A.part + ((long)n * A.nblk + blockIdx.x) * 4;
+    o[0] = sh[0][0]; o[1] = sh[1][0]; o[2] = sh[2][0]; o[3] = sh[3][0];
+  }
+}
+
+// Adds the block partials of every tracer in index order and runs the scalar steps that
+// follow each global sum (one lane per tracer).
+template <int PASS>
+__global__ void k_bc_fin(BcArgs A) {
+  int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= A.ntr) return;
+  DD s0{0.0, 0.0}, s1{0.0, 0.0};
+  const double* pp = A.part + (long)n * A.nblk * 4;
+  for (int b = 0; b < A.nblk; b++) {
+    s0 = dd_merge(s0, DD{pp[4 * b], pp[4 * b + 1]});
+    s1 = dd_merge(s1, DD{pp[4 * b + 2], pp[4 * b + 3]});
+  }
+  double sum0 = s0.hi + s0.lo + 0.0, sum1 = s1.hi + s1.lo + 0.0;
+  double* sc = A.sc + n * kSc;
+  if (PASS == P_PM) {
+    // Bermejo_Conde.F90:92-105 (mass fluxes are zero: Aranami flux is not applied)
+    double mass_p = sum0 - A.deficit[n];
+    double mass_bflux = 0.0 - 0.0;
+    sc[SC_MP] = mass_p;
+    sc[SC_MM] = sum1;
+    sc[SC_MTP] = mass_p + mass_bflux;
+    sc[SC_D] = sum1 - sc[SC_MTP];
+    sc[SC_CYCLE] = 0.0;
+  } else if (PASS == P_W) {
+    sc[SC_W] = sum0;
+    if (d_almost_zero(sum0)) sc[SC_CYCLE] = 1.0;               // :190-198
+    else sc[SC_LAMBDA] = sc[SC_D] / sum0;                       // :200
+  } else if (PASS == P_CORR && A.min_max) {
+    sc[SC_MBCR] = sum0;
+    if (sc[SC_CYCLE] == 0.0) sc[SC_PROP] = sc[SC_MTP] / sum0;   // :273
+  } else {  // final mass: P_PROP, or P_CORR without min/max
+    sc[SC_MBC] = sum0;
+    if (sc[SC_CYCLE] == 0.0) A.deficit[n] = sum0 - sc[SC_MTP];  // :298-303
+    if (A.diag) {
+      double* d = A.diag + 6 * n;
+      d[0] = sc[SC_MTP]; d[1] = sc[SC_MM]; d[2] = sc[SC_W]; d[3] = sum0; d[4] = sc[SC_CYCLE]; d[5] = A.deficit[n];
+    }
+  }
+}
+
+}  // namespace adz
+
+using namespace adz;
+
+struct adz_bc {
+  int tdtype;
+  int has_w3;  // some tracer uses weight 3 (needs pkps)
+  BcArgs a;
+  void* mem;
+};
+
+extern "C" {
+
+int adz_bc_create(const adz_bc_tracer* tr, int ntr, const adz_bc_domain* dom, int tdtype, adz_bc** out) {
+  if (!tr || ntr <= 0 || !dom || !out) return fail(ADZ_EINVAL, "adz_bc_create: bad argument");
+  if (tdtype != ADZ_F32 && tdtype != ADZ_F64) return fail(ADZ_EINVAL, "adz_bc_create: tdtype must be F32 or F64");
+  const adz_bc_domain& d = *dom;
+  if (d.in < d.i0 || d.jn < d.j0 || d.nk < d.k0 || d.i0 < d.minx || d.in > d.maxx || d.j0 < d.miny ||
+      d.jn > d.maxy || d.k0 < 1)
+    return fail(ADZ_EINVAL, "adz_bc_create: scope outside the arrays");
+  std::vector<BcTr> h(ntr);
+  for (int n = 0; n < ntr; n++) {
+    const adz_bc_tracer& t = tr[n];
+    if (!t.p || !t.m || !t.lin || !t.mn || !t.mx) return fail(ADZ_EINVAL, "adz_bc_create: null tracer array");
+    if (t.weight < 1 || t.weight > 3) return fail(ADZ_EINVAL, "BERMEJO-CONDE: BC WEIGHT not valid");
+    if (!t.legacy && (t.pexp_n < 1 || t.pexp_n > 9)) return fail(ADZ_EINVAL, "BERMEJO-CONDE: BC PEXP not valid");
+    h[n] = BcTr{t.p, t.m, t.lin, t.mn, t.mx, t.legacy, t.weight,
+                t.legacy ? 1.0 : (double)d.pexp_list[t.pexp_n - 1]};
+  }
+  BcArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.ntr = ntr;
+  a.nxa = d.maxx - d.minx + 1; a.nya = d.maxy - d.miny + 1;
+  a.i0 = d.i0; a.j0 = d.j0; a.k0 = d.k0; a.nx = d.in - d.i0 + 1; a.ny = d.jn - d.j0 + 1;
+  a.npts = (long)a.nx * a.ny * (d.nk - d.k0 + 1);
+  a.min_max = d.min_max;
+  long per = (long)kBcBlock * 16;
+  long nblk = (a.npts + per - 1) / per;
+  if (nblk > 2048) nblk = 2048;
+  if (nblk < 1) nblk = 1;
+  a.nblk = (int)nblk;
+  a.chunk = (a.npts + nblk - 1) / nblk;
+  size_t tb = sizeof(BcTr) * ntr, pb = sizeof(double) * 4 * ntr * nblk, sb = sizeof(double) * kSc * ntr;
+  void* mem = nullptr;
+  if (hipMalloc(&mem, tb + pb + sb) != hipSuccess) return fail(ADZ_ENOMEM, "adz_bc_create: hipMalloc");
+  if (hipMemcpy(mem, h.data(), tb, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemset((char*)mem + tb, 0, pb + sb) != hipSuccess) {
+    (void)hipFree(mem);
+    return fail(ADZ_EHIP, "adz_bc_create: upload");
+  }
+  a.tr = (const BcTr*)mem;
+  a.part = (double*)((char*)mem + tb);
+  a.sc = (double*)((char*)mem + tb + pb);
+  adz_bc* b = new adz_bc;
+  b->tdtype = tdtype; b->a = a; b->mem = mem;
+  b->has_w3 = 0;
+  for (int n = 0; n < ntr; n++) b->has_w3 |= (tr[n].weight == 3);
+  *out = b;
+  return ADZ_OK;
+}
+
+int adz_bermejo_conde(adz_bc* bc, const void* airm1, const void* airm0, const void* pkps, const double* mask,
+                      double* deficit, double* diag, adz_stream stream) {
+  if (!bc || !airm1 || !airm0 || !mask || !deficit) return fail(ADZ_EINVAL, "adz_bermejo_conde: null argument");
+  BcArgs a = bc->a;
+  a.airm1 = airm1; a.airm0 = airm0; a.pkps = pkps; a.mask = mask; a.deficit = deficit; a.diag = diag;
+  if (bc->has_w3 && !pkps) return fail(ADZ_EINVAL, "adz_bermejo_conde: weight 3 needs pkps");
+  hipStream_t st = (hipStream_t)stream;
+  dim3 g((unsigned)a.nblk, (unsigned)a.ntr), b(kBcBlock);
+  dim3 gf((unsigned)((a.ntr + 63) / 64)), bf(64);
+#define ADZ_BC_PASSES(T)                                                       \
+  hipLaunchKernelGGL((k_bc_pass<T, P_PM>), g, b, 0, st, a);                   \
+  hipLaunchKernelGGL((k_bc_fin<P_PM>), gf, bf, 0, st, a);                     \
+  hipLaunchKernelGGL((k_bc_pass<T, P_W>), g, b, 0, st, a);                    \
+  hipLaunchKernelGGL((k_bc_fin<P_W>), gf, bf, 0, st, a);                      \
+  hipLaunchKernelGGL((k_bc_pass<T, P_CORR>), g, b, 0, st, a);                 \
+  hipLaunchKernelGGL((k_bc_fin<P_CORR>), gf, bf, 0, st, a);                   \
+  if (a.min_max) {                                                            \
+    hipLaunchKernelGGL((k_bc_pass<T, P_PROP>), g, b, 0, st, a);               \
+    hipLaunchKernelGGL((k_bc_fin<P_PROP>), gf, bf, 0, st, a);                 \
+  }
+  if (bc->tdtype == ADZ_F32) {
+    ADZ_BC_PASSES(float)
+  } else {
+    ADZ_BC_PASSES(double)
+  }
+#undef ADZ_BC_PASSES
+  ADZ_HIP_CHECK(hipGetLastError());
+  return ADZ_OK;
+}
+
+int adz_bc_free(adz_bc* bc) {
+  if (!bc) return ADZ_OK;
+  hipError_t e = hipFree(bc->mem);
+  delete bc;
+  if (e != hipSuccess) return fail(ADZ_EHIP, "adz_bc_free: hipFree");
+  return ADZ_OK;
+}
+
+}  // extern "C"

@@ -220,6 +220,48 @@ int adz_plan_execute(const adz_plan* p, adz_stream stream);
 int adz_plan_tune(const adz_plan* p, adz_tune* out); /* the parameters in use */
 int adz_plan_free(adz_plan* p);
 
+/* ------------------------------------------------------------------------------------
+ * Level 4: Bermejo-Conde mass fixer (Bermejo_Conde.F90:18-320 with the sums of
+ * mass_tr_PM.F90 / mass_tr_X.F90) on device arrays.
+ * ----------------------------------------------------------------------------------*/
+
+/* One tracer registered for the fixer (type bc, adz_mem.F90:109-125).  DEVICE pointers to
+ * element (1,1,1) of model-layout arrays of the fixer's storage type. */
+typedef struct adz_bc_tracer {
+  const void* p;   /* tracer at time P (tracers_P)                            */
+  void* m;         /* tracer at time M after interpolation, corrected in place */
+  const void* lin; /* Adz_post lin (already clipped to [min,max])             */
+  const void* mn;  /* Adz_post min                                            */
+  const void* mx;  /* Adz_post max                                            */
+  int legacy;      /* mass = 1 (additive, p = 1)                              */
+  int weight;      /* 1 additive, 2 multiplicative, 3 additive * pr_k/pr_s    */
+  int pexp_n;      /* index (1-based) into pexp_list when not legacy          */
+} adz_bc_tracer;
+
+typedef struct adz_bc_domain {
+  int minx, maxx, miny, maxy, nk; /* array layout (l_minx:l_maxx, l_miny:l_maxy, nk)   */
+  int i0, in, j0, jn, k0;         /* scope of the sums and corrections (core, k0t:nk)  */
+  int min_max;                    /* Adz_BC_min_max_L                                  */
+  float pexp_list[9];             /* Adz_BC_pexp_list                                  */
+} adz_bc_domain;
+
+typedef struct adz_bc adz_bc;
+/* Register ntr tracers (tdtype ADZ_F32 = the reference's `real` arrays, or ADZ_F64) and
+ * allocate the fixer's scratch (per-block FP64 partial sums, per-tracer scalars) once. */
+int adz_bc_create(const adz_bc_tracer* tr, int ntr, const adz_bc_domain* dom, int tdtype, adz_bc** out);
+/* Apply the fixer to every registered tracer.  airm1/airm0 (air mass at time P / M) and
+ * pkps (pr_k/pr_s, weight 3 only; may be NULL otherwise) are DEVICE arrays of the model
+ * layout; mask is the DEVICE FP64 (minx:maxx, miny:maxy) geomh_mask_8; deficit is a DEVICE
+ * FP64 array [ntr] holding BC_mass_deficit, read and updated (the carried epsilon,
+ * Bermejo_Conde.F90:100,292-308).  diag (DEVICE [6*ntr] or NULL) receives, per tracer:
+ * target mass, mass before correction, weight mass, final mass, skipped flag, deficit.
+ * Global FP64 sums are computed in a fixed order with double-double partials, so the
+ * result is deterministic and equals the exactly rounded sum (the reference sums each
+ * rank sequentially, mass_tr_X.F90:86-93).  No host synchronisation: capturable. */
+int adz_bermejo_conde(adz_bc* bc, const void* airm1, const void* airm0, const void* pkps, const double* mask,
+                      double* deficit, double* diag, adz_stream stream);
+int adz_bc_free(adz_bc* bc);
+
 #ifdef __cplusplus
 }
 #endif

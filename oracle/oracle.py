@@ -301,3 +301,48 @@ class Grid:
         if getattr(self, "h", None) and _lib is not None:
             _lib.ora_grid_free(self.h)
             self.h = None
+
+
+def bermejo_conde(tracers, layout, scope, airm1, airm0, mask, deficit, pkps=None, min_max=True,
+                  pexp_list=(1.0,) * 9, sum_mode=1):
+    """Bermejo_Conde restatement on host arrays (array-start layout).  tracers: list of
+    dicts with p, m (modified in place), lin, min, max and `mass` code.  sum_mode 1 =
+    exactly rounded sums, 0 = the reference's sequential order.  Returns (deficit, diag)."""
+    minx, maxx, miny, maxy, nk = layout
+    i0, in_, j0, jn, k0 = scope
+    ntr = len(tracers)
+    dt = tracers[0]["m"].dtype
+    tdt = 1 if dt == np.float32 else 0
+
+    def ptrs(key):
+        arr = (C.c_void_p * ntr)()
+        for n, t in enumerate(tracers):
+            arr[n] = t[key].ctypes.data
+        return arr
+
+    def codes(mass):
+        if mass == 1:
+            return 1, 1, 0
+        w = (mass - 100) // 10
+        return 0, w, mass - (100 + w * 10)
+
+    cs = [codes(t["mass"]) for t in tracers]
+    legacy = np.array([c[0] for c in cs], np.int32)
+    weight = np.array([c[1] for c in cs], np.int32)
+    pexp_n = np.array([c[2] for c in cs], np.int32)
+    dom = np.array([minx, maxx, miny, maxy, nk, i0, in_, j0, jn, k0, 1 if min_max else 0, sum_mode], np.int32)
+    de = np.ascontiguousarray(deficit, dtype=np.float64).copy()
+    diag = np.zeros(6 * ntr)
+    pk = pkps if pkps is not None else np.zeros_like(airm0)
+    pl = np.asarray(pexp_list, np.float32)
+    lib().ora_bermejo_conde(tdt, _p(dom), ntr, ptrs("p"), ptrs("m"), ptrs("lin"), ptrs("min"), ptrs("max"),
+                            _p(legacy), _p(weight), _p(pexp_n), _p(de), _p(airm1), _p(airm0), _p(pk), _p(mask),
+                            _p(pl), _p(diag))
+    return de, diag.reshape(ntr, 6)
+
+
+def mass(x, air, mask, layout, scope, sum_mode=1):
+    minx, maxx, miny, maxy, nk = layout
+    i0, in_, j0, jn, k0 = scope
+    dom = np.array([minx, maxx, miny, maxy, nk, i0, in_, j0, jn, k0, 0, sum_mode], np.int32)
+    return lib().ora_mass(1 if x.dtype == np.float32 else 0, _p(dom), _p(x), _p(air), _p(mask))

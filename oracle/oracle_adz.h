@@ -648,12 +648,11 @@ inline void bermejo_conde(const BCDomain& D, std::vector<BCTracer<T>>& tr, const
   }
   for (int n = 0; n < ntr; n++) {
     double mass_bc = mass_of(D, (const T*)tr[n].m, airm0);
+    if (cycle_tr[n] != 1) tr[n].mass_deficit = mass_bc - mass_tot_p[n];
     if (diag) {
       diag[6 * n + 0] = mass_tot_p[n]; diag[6 * n + 1] = mass_m[n]; diag[6 * n + 2] = mass_wei[n];
-      diag[6 * n + 3] = mass_bc; diag[6 * n + 4] = (double)cycle_tr[n]; diag[6 * n + 5] = mass_deficit[n];
+      diag[6 * n + 3] = mass_bc; diag[6 * n + 4] = (double)cycle_tr[n]; diag[6 * n + 5] = tr[n].mass_deficit;
     }
-    if (cycle_tr[n] == 1) continue;
-    tr[n].mass_deficit = mass_bc - mass_tot_p[n];
   }
 }
 
