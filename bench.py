@@ -1,0 +1,392 @@
+#!/usr/bin/env python3
+"""Benchmark of the semi-Lagrangian advection step on one MI355X.
+
+Workload (default `c5`): 2048x1024x84 single-tile LAM grid, halo 5, dt = 450 s, winds
+with Courant numbers up to 6 (x, y) and 2 (z); 64 interpolated fields: u at the u-grid
+positions, v at the v-grid positions, T, q and 60 tracers at the thermodynamic positions;
+the Bermejo-Conde fixer on the 60 tracers.  One step = trajectory (adz_traject, FP64
+positions) + batched FP32 interpolation (persistent plans) + Bermejo-Conde, captured in
+a hipGraph and replayed.  Inputs are synthetic and already resident in HBM.
+
+Printed JSON line: value = interpolated Gpoints/s of the whole step (points x fields /
+step time).  Beside it: the interpolation-kernel-only rate and HBM GB/s, the plain FP64
+path's rate at the same positions (the FP64 drop-in), the per-field max / RMS relative
+error of the FP32 path against the FP64 path (bit-exact with the FP64 oracle at test
+sizes), the roofline of the dominant interpolation kernel, and the reference CPU path
+timed on this host on a bounded sample.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+MH = 2          # model halo of destination / tracer arrays
+HBM_PEAK = 8000.0  # GB/s, MI355X published HBM3E bandwidth
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=5)
+    p.add_argument("--warmup", type=int, default=2)
+    p.add_argument("--config", default="c5")
+    p.add_argument("--bc-group", type=int, default=20, help="tracers per Bermejo-Conde group")
+    p.add_argument("--fp64-fields", type=int, default=16, help="fields in the plain FP64 timing leg")
+    p.add_argument("--err-fields", type=int, default=0, help="fields checked against FP64 (0 = all)")
+    p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--cpu-seconds", type=float, default=12.0)
+    p.add_argument("--profile-kernel", action="store_true",
+                   help="only build the workload and run the dominant interpolation kernel 5 times (rocprofv3)")
+    return p.parse_args()
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU: the path shards by horizontal domain "
+                                   "decomposition (one tile per GPU), which is not part of this build; "
+                                   "run with --gpus 1"}))
+        sys.exit(2)
+    import numpy as np
+    import torch
+
+    from gem_amd import _lib as L
+    from gem_amd import adz as A
+    from gem_amd import synth as S
+
+    assert torch.cuda.is_available(), "bench.py needs a HIP device"
+    dev = torch.device("cuda:0")
+    cfg = S.CONFIGS[args.config]
+    g = S.grid_desc(cfg)
+    nf = cfg.nfields
+    ntr = cfg.ntracers
+    t_setup = time.time()
+
+    # ------------------------------------------------------------------ state
+    ctx = A.AdzContext(g, L.ADZ_F64, itraj=cfg.niter)
+    info = ctx.info
+    w = {k: v.to(dev) for k, v in S.winds(cfg, g, device=dev).items()}
+    ext_shape = (cfg.nk, cfg.nj + 2 * cfg.halo, cfg.ni + 2 * cfg.halo)
+    mod_shape = (cfg.nk, cfg.nj + 2 * MH, cfg.ni + 2 * MH)
+    n_ext = math.prod(ext_shape)
+    n_mod = math.prod(mod_shape)
+    ext_off = (1 - info["lminx"]) + info["nit"] * (1 - info["lminy"])
+    mod_layout = (1 - MH, cfg.ni + MH, 1 - MH, cfg.nj + MH)
+    mod_off = MH + (cfg.ni + 2 * MH) * MH
+    nb = min(args.bc_group, ntr) if ntr else 0
+    est = (3 * n_ext + (nf - 3) * n_mod + nf * n_mod + 3 * nb * n_mod) * 4 / 1e9
+    log(f"config {cfg.name}: {cfg.ni}x{cfg.nj}x{cfg.nk}, {nf} fields, est. field memory {est:.1f} GB")
+    if est > 230:
+        raise SystemExit("workload would not leave headroom in HBM")
+
+    # sources: u, v, T on the extended layout (orhs*_ext), q and tracers on the model layout
+    def gen_field(q, dtype):
+        if q < 3:
+            f = S.fields(cfg, 1, device=dev, kind="smooth", seed=cfg.seed + 31 * q)[0]
+            f = f.reshape(ext_shape)
+            scale, off = [(20.0, 0.0), (20.0, 0.0), (15.0, 250.0)][q]
+            return (off + scale * f / 8.0).reshape(-1).to(dtype)
+        shape = mod_shape
+        rng_seed = cfg.seed + 31 * q
+        if q == 3:
+            f = S.smooth_field(shape, np.random.default_rng(rng_seed), dev)
+            return (0.006 + 0.002 * f / 8.0).reshape(-1).to(dtype)
+        return S.plume_tracer(shape, np.random.default_rng(rng_seed), dev).reshape(-1).to(dtype)
+
+    src = [gen_field(q, torch.float32) for q in range(nf)]
+    dst = [torch.zeros(n_mod, dtype=torch.float32, device=dev) for _ in range(nf)]
+    post = [{k: torch.zeros(n_mod, dtype=torch.float32, device=dev) for k in ("lin", "min", "max")}
+            for _ in range(nb)]
+    airm1, airm0, pkps, mask = S.airmass(cfg, g, device=dev, mh=MH)
+    airm1, airm0, pkps = airm1.reshape(-1).contiguous(), airm0.reshape(-1).contiguous(), pkps.reshape(-1).contiguous()
+    mask = mask.reshape(-1).contiguous()
+    deficit = torch.zeros(max(ntr, 1), dtype=torch.float64, device=dev)
+
+    ctx.adz_inittraj()
+    step_traj = lambda: ctx.adz_traject(cfg.dt, w["uu_ext"], w["vv_ext"], w["ww_ext"], w["ut0"], w["vt0"],
+                                        w["zdt0"])
+    step_traj()
+    torch.cuda.synchronize()
+    pmu, pmv, pt = ctx.array(A.ADZ_A_PMU), ctx.array(A.ADZ_A_PMV), ctx.array(A.ADZ_A_PT)
+    boxes = {
+        "u": ((info["i0u"], info["inu"], info["j0"], info["jn"], info["k0"], cfg.nk), pmu, ctx.ztab(False)),
+        "v": ((info["i0"], info["in"], info["j0v"], info["jnv"], info["k0"], cfg.nk), pmv, ctx.ztab(False)),
+        "t": ((info["i0"], info["in"], info["j0"], info["jn"], info["k0t"], cfg.nk), pt, ctx.ztab(True)),
+    }
+
+    def npts(box):
+        i0, in_, j0, jn, k0, nk = box
+        return (in_ - i0 + 1) * (jn - j0 + 1) * (nk - k0 + 1)
+
+    def item(name, fields, ext):
+        box, xyz, zt = boxes[name]
+        so = ext_off if ext else mod_off
+        return dict(geom=zt, xyz=xyz, box=box, dlayout=mod_layout, ext=ext,
+                    fields=A.field_list(fields, so, mod_off))
+
+    # plan A: u, v, T (extended sources) and q (model layout), no post-processing
+    itemsA = [item("u", [dict(src=src[0], dst=dst[0])], True), item("v", [dict(src=src[1], dst=dst[1])], True),
+              item("t", [dict(src=src[2], dst=dst[2])], True),
+              item("t", [dict(src=src[3], dst=dst[3])], False)]
+    planA = A.Plan(itemsA, L.ADZ_F32, math=L.ADZ_MATH_F32, post=0)
+    # plans B: tracer groups with lin/min/max for the fixer, and the fixer itself
+    groups = []
+    tbox = boxes["t"][0]
+    scope = (1 + cfg.pil, cfg.ni - cfg.pil, 1 + cfg.pil, cfg.nj - cfg.pil, info["k0t"])  # core (adz_post_tr.F90:141)
+    layout5 = mod_layout + (cfg.nk,)
+    for g0 in range(4, nf, nb if nb else nf):
+        ids = list(range(g0, min(nf, g0 + nb)))
+        stack = [dict(src=src[q], dst=dst[q], lin=post[k]["lin"], min=post[k]["min"], max=post[k]["max"])
+                 for k, q in enumerate(ids)]
+        plan = A.Plan([item("t", stack, False)], L.ADZ_F32, math=L.ADZ_MATH_F32, post=A.ADZ_POST_STORE)
+        bcs = [dict(p=src[q], m=dst[q], lin=post[k]["lin"], min=post[k]["min"], max=post[k]["max"], mass=1)
+               for k, q in enumerate(ids)]
+        bc = A.BermejoConde(bcs, layout5, scope, mod_off, min_max=True)
+        groups.append((ids, plan, bc))
+    pts_total = npts(boxes["u"][0]) + npts(boxes["v"][0]) + (nf - 2) * npts(tbox)
+    log(f"setup {time.time() - t_setup:.1f}s; {pts_total / 1e9:.3f} G interpolated point-fields per step")
+
+    def interp():
+        planA.execute()
+        for ids, plan, bc in groups:
+            plan.execute()
+
+    def step():
+        step_traj()
+        planA.execute()
+        for ids, plan, bc in groups:
+            plan.execute()
+            bc(airm1, airm0, mask, deficit[ids[0] - 4:ids[-1] - 3], pkps=pkps)
+
+    ev = lambda: torch.cuda.Event(enable_timing=True)
+
+    def timed(fn, reps):
+        torch.cuda.synchronize()
+        a, b = ev(), ev()
+        a.record()
+        for _ in range(reps):
+            fn()
+        b.record()
+        torch.cuda.synchronize()
+        return a.elapsed_time(b) / reps
+
+    if args.profile_kernel:
+        ids, plan, bc = groups[0]
+        for _ in range(5):
+            plan.execute()
+        torch.cuda.synchronize()
+        # calibration copy of known size for the PMC byte counters (1 GiB read + 1 GiB write)
+        x = torch.empty(2 ** 28, dtype=torch.float32, device=dev)
+        y = torch.empty_like(x)
+        y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"profile_kernel": "k_plan (tracer group)", "copy_bytes": 2 * 4 * 2 ** 28}), flush=True)
+        return
+
+    # ------------------------------------------------------------------ timed step (hipGraph)
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    eager_ms = timed(step, 1)
+    graph = torch.cuda.CUDAGraph()
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        step()  # warm on the capture stream
+        torch.cuda.synchronize()
+        with torch.cuda.graph(graph, stream=s):
+            step()
+    torch.cuda.current_stream().wait_stream(s)
+    graph.replay()
+    torch.cuda.synchronize()
+    t0 = time.time()
+    torch.cuda.synchronize()
+    a, b = ev(), ev()
+    a.record()
+    for _ in range(args.steps):
+        graph.replay()
+    b.record()
+    torch.cuda.synchronize()
+    ms_step = a.elapsed_time(b) / args.steps
+    wall = time.time() - t0
+    log(f"step: graph {ms_step:.2f} ms, eager {eager_ms:.2f} ms ({args.steps} replays in {wall:.2f}s)")
+
+    # ------------------------------------------------------------------ phase breakdown
+    t_traj = timed(step_traj, 3)
+    t_interp = timed(interp, 3)
+    ids0, plan0, bc0 = groups[0] if groups else (None, None, None)
+    t_plan0 = timed(plan0.execute, 5) if plan0 else float("nan")
+    t_bc = sum(timed(lambda: bc(airm1, airm0, mask, deficit[i[0] - 4:i[-1] - 3], pkps=pkps), 2)
+               for i, _, bc in groups)
+    log(f"phases: traject {t_traj:.2f} ms, interpolation {t_interp:.2f} ms (tracer-group kernel {t_plan0:.2f} ms), "
+        f"Bermejo-Conde {t_bc:.2f} ms")
+
+    # algorithmic bytes of the tracer-group kernel (one launch): sources + targets + lin/min/max
+    # (3 extra FP32 outputs) + FP64 positions once
+    Nt = npts(tbox)
+    ng = len(ids0) if ids0 else 0
+    alg_plan0 = ng * 4 * (n_mod + Nt) + ng * 3 * 4 * Nt + 24 * Nt
+    achieved = alg_plan0 / (t_plan0 * 1e-3) / 1e9
+    # interpolation-only throughput over all 64 fields
+    alg_interp = (3 * 4 * n_ext + (nf - 3) * 4 * n_mod + 4 * pts_total + ntr * 12 * Nt +
+                  24 * (npts(boxes["u"][0]) + npts(boxes["v"][0]) + Nt))
+    interp_gpts = pts_total / (t_interp * 1e-3) / 1e9
+
+    # ------------------------------------------------------------------ accuracy vs FP64
+    interp()  # dst now holds the pure FP32 interpolation (no fixer)
+    torch.cuda.synchronize()
+    check = range(nf) if args.err_fields <= 0 else range(min(nf, args.err_fields))
+    errs = {}
+    worst_rms, worst_max = 0.0, 0.0
+    d64 = torch.zeros(n_mod, dtype=torch.float64, device=dev)
+    for q in check:
+        s64 = gen_field(q, torch.float64)
+        name = "u" if q == 0 else ("v" if q == 1 else "t")
+        box, xyz, zt = boxes[name]
+        d64.zero_()
+        A.adz_tricub_hlt([dict(src=s64, dst=d64)], xyz, zt, *box[:5], box[5], mod_layout, ext=q < 3,
+                         math=L.ADZ_MATH_REF, src_off=ext_off if q < 3 else mod_off)
+        i0, in_, j0, jn, k0, nk = box
+        a64 = d64.reshape(mod_shape)[k0 - 1:, j0 - 1 + MH:jn + MH, i0 - 1 + MH:in_ + MH]
+        a32 = dst[q].reshape(mod_shape)[k0 - 1:, j0 - 1 + MH:jn + MH, i0 - 1 + MH:in_ + MH].double()
+        diff = a32 - a64
+        rms = float(torch.linalg.vector_norm(diff) / torch.linalg.vector_norm(a64))
+        mx = float(diff.abs().max() / a64.abs().max())
+        errs[str(q)] = [mx, rms]
+        worst_rms, worst_max = max(worst_rms, rms), max(worst_max, mx)
+        del s64
+    log(f"FP32 vs FP64: worst max rel {worst_max:.3e}, worst RMS rel {worst_rms:.3e} over {len(errs)} fields")
+
+    # ------------------------------------------------------------------ plain FP64 leg
+    nfp = min(args.fp64_fields, nf - 4) if nf > 4 else 1
+    del post
+    torch.cuda.empty_cache()
+    s64 = [gen_field(4 + q, torch.float64) for q in range(nfp)]
+    d64s = [torch.zeros(n_mod, dtype=torch.float64, device=dev) for _ in range(nfp)]
+    stack64 = [dict(src=a, dst=b) for a, b in zip(s64, d64s)]
+    box, xyz, zt = boxes["t"]
+    run64 = lambda: A.adz_tricub_hlt(stack64, xyz, zt, *box[:5], box[5], mod_layout, ext=False,
+                                     math=L.ADZ_MATH_REF, src_off=mod_off)
+    run64()
+    t64 = timed(run64, 3)
+    plain_gpts = nfp * Nt / (t64 * 1e-3) / 1e9
+    # the FP32 form on the same fields, same positions, same entry point
+    stack32 = [dict(src=src[4 + q], dst=dst[4 + q]) for q in range(nfp)]
+    run32 = lambda: A.adz_tricub_hlt(stack32, xyz, zt, *box[:5], box[5], mod_layout, ext=False,
+                                     math=L.ADZ_MATH_F32, src_off=mod_off)
+    run32()
+    t32 = timed(run32, 3)
+    f32_gpts = nfp * Nt / (t32 * 1e-3) / 1e9
+    log(f"plain FP64: {plain_gpts:.1f} Gpt/s; FP32 same call: {f32_gpts:.1f} Gpt/s ({nfp} fields)")
+    del s64, d64s, stack64
+    torch.cuda.empty_cache()
+
+    # ------------------------------------------------------------------ CPU baseline
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(np, torch, cfg, g, src, pt, tbox, mod_shape, nf, args.cpu_seconds)
+
+    value = pts_total / (ms_step * 1e-3) / 1e9
+    out = {
+        "metric": "interpolated Gpoints/s + HBM GB/s: FP32 batched vs FP64 plain at 1e-5 RMS",
+        "value": round(value, 3),
+        "unit": "Gpoints/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded analytic winds, sinusoid and plume fields, generated in HBM)",
+        "config": {"workload": f"{cfg.name}: {cfg.ni}x{cfg.nj}x{cfg.nk} LAM tile, halo {cfg.halo}, dt {cfg.dt:g} s, "
+                               f"{nf} fields (u, v, T, q + {ntr} tracers), FP32 storage, FP64 positions, "
+                               f"Bermejo-Conde on {ntr} tracers, one step = trajectory + interpolation + fixer "
+                               f"in one hipGraph",
+                   "points_x_fields_per_step": pts_total, "bc_group": nb},
+        "phases_ms": {"traject": round(t_traj, 3), "interpolation": round(t_interp, 3),
+                      "bermejo_conde": round(t_bc, 3), "eager_step": round(eager_ms, 3)},
+        "interp_gpoints_per_s": round(interp_gpts, 3),
+        "interp_alg_GBps": round(alg_interp / (t_interp * 1e-3) / 1e9, 1),
+        "fp64_plain_gpoints_per_s": round(plain_gpts, 3),
+        "fp32_same_call_gpoints_per_s": round(f32_gpts, 3),
+        "speedup_fp32_vs_fp64_plain": round(f32_gpts / plain_gpts, 3),
+        "error_vs_fp64": {"worst_max_rel": worst_max, "worst_rms_rel": worst_rms, "per_field_max_rms": errs},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK, 4), "traffic": None,
+                     "kernel": f"k_plan<float,double,F32,post> tracer group ({ng} fields), "
+                               f"{t_plan0:.3f} ms/launch, {alg_plan0 / 1e9:.2f} GB algorithmic"},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def cpu_baseline(np, torch, cfg, g, src, pt, tbox, mod_shape, nf, seconds):
+    """The reference CPU interpolator timed on this host on a bounded sample of the same
+    workload: 8 tracers at the thermodynamic departure points of 12 mid-levels.  Uses the
+    reference's turbo_tricublin.c build (oracle/_ref) when present, else the oracle port."""
+    import concurrent.futures as cf
+
+    from oracle import oracle as O
+
+    i0, in_, j0, jn, k0, nk = tbox
+    nx, ny = in_ - i0 + 1, jn - j0 + 1
+    klo, khi = 36, 47                      # target levels of the sample
+    plo, phi = klo - 6, khi + 6            # source planes copied (displacement <= 2 + stencil + margin)
+    nfs = min(8, nf - 4)
+    pos = pt.reshape(nk - k0 + 1, ny, nx, 3)[klo - k0:khi - k0 + 1].cpu().numpy().astype(np.float32)
+    pos[..., 2] -= (plo - 1)
+    assert pos[..., 2].min() >= 2.0 and pos[..., 2].max() <= phi - plo - 1.0
+    fields = [np.ascontiguousarray(src[4 + q].reshape(mod_shape)[plo - 1:phi].cpu().numpy()).reshape(-1)
+              for q in range(nfs)]
+    levels = g["zt"][plo:phi + 1]
+    ni_m, nj_m = mod_shape[2], mod_shape[1]
+    base = MH + ni_m * MH
+    nthreads = max(1, min(16, os.cpu_count() or 1))
+    slices = [np.ascontiguousarray(pos[k].reshape(-1)) for k in range(pos.shape[0])]
+    npts = sum(s.size // 3 for s in slices)
+    try:
+        R = O.RefKernel("avx2")
+        lv = R.setup(levels, ni_m, nj_m, 0, 0)
+        kind = "reference"
+
+        def work(s):
+            return R.zyx1_m_n(lv, fields, base, s)
+    except (FileNotFoundError, OSError):
+        R, kind = None, "port"
+        zt = O.Ztab(levels, ni_m, nj_m, 0, 0)
+
+        def work(s):
+            return O.zyx1_m_n(zt, fields, base, s)
+    reps, t0 = 0, time.time()
+    if kind == "reference":
+        with cf.ThreadPoolExecutor(nthreads) as ex:  # one work item per level slice, as the OpenMP do
+            while time.time() - t0 < seconds:
+                list(ex.map(work, slices))
+                reps += 1
+    else:
+        os.environ.setdefault("OMP_NUM_THREADS", str(nthreads))
+        allpos = np.concatenate(slices)
+        while time.time() - t0 < seconds:
+            work(allpos)
+            reps += 1
+    dt = time.time() - t0
+    rate = reps * npts * nfs / dt / 1e9
+    log(f"cpu baseline ({kind}, {nthreads} threads): {rate:.3f} Gpt/s")
+    return {"value": round(rate, 4), "unit": "Gpoints/s", "cores": nthreads, "kind": kind,
+            "sample": f"{nfs} tracers x {npts} departure points (levels {klo}-{khi} of {cfg.name}), "
+                      f"Tricublin_zyx1_m_n per level slice, FP32 fields, {reps} passes in {dt:.1f} s"}
+
+
+if __name__ == "__main__":
+    main()

@@ -277,3 +277,36 @@ def fields(cfg, n, device="cpu", kind="smooth", seed=None, dtype=torch.float64):
             raise ValueError(kind)
         out.append(f.reshape(-1).to(dtype))
     return out
+
+
+# ------------------------------------------------------------------ air mass (BC inputs)
+def airmass(cfg, g, device="cpu", mh=2, seed=None):
+    """Air mass at time P and M, pr_t/pr_s and the area mask on the model layout (halo mh),
+    formed as get_air_mass_hlt.F90:83-98 does: (p_{k+1} - p_k) / g * area (dry air, sumq=0),
+    with hybrid pressures p_k = exp(zeta_k) * (ps/pref)**b_k, b_k the normalised height.
+    Returns FP32 airm1, airm0, pkps (nk, nj+2mh, ni+2mh) and the FP64 mask (nj+2mh, ni+2mh)."""
+    rng = np.random.default_rng((cfg.seed if seed is None else seed) + 104729)
+    nk = cfg.nk
+    ny, nx = cfg.nj + 2 * mh, cfg.ni + 2 * mh
+    zm = torch.tensor(g["zm"], dtype=torch.float64, device=device)
+    zt = torch.tensor(g["zt"], dtype=torch.float64, device=device)
+    b = ((zm - zm[0]) / (zm[nk + 1] - zm[0]))[:, None, None]
+    bt = ((zt - zt[0]) / (zt[nk + 1] - zt[0]))[:, None, None]
+    j = torch.arange(1 - mh, cfg.nj + mh + 1, device=device, dtype=torch.float64)
+    i = torch.arange(1 - mh, cfg.ni + mh + 1, device=device, dtype=torch.float64)
+    lat = g["lat0"] + (j - 1) * g["h"]
+    area = (RAYT * g["h"]) ** 2 * torch.cos(lat)[:, None] * torch.ones(1, nx, device=device, dtype=torch.float64)
+    out = []
+    for t in range(2):
+        ph = rng.uniform(0, 2 * math.pi, 2)
+        ps = PREF * (1.0 + 0.03 * torch.sin(2 * math.pi * i[None, :] / nx + ph[0]) *
+                     torch.cos(2 * math.pi * j[:, None] / ny + ph[1]))
+        p = torch.exp(zm)[:, None, None] * (ps / PREF)[None] ** b           # levels 0..nk+1
+        p[nk + 1] = ps
+        airm = (p[2:nk + 2] - p[1:nk + 1]) / GRAV * area[None]               # k = 1..nk
+        out.append(airm.float())
+        if t == 0:
+            pt = torch.exp(zt)[:, None, None] * (ps / PREF)[None] ** bt
+            pkps = (pt[1:nk + 1] / ps[None]).float()
+    mask = torch.ones(ny, nx, dtype=torch.float64, device=device)
+    return out[1], out[0], pkps, mask
