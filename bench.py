@@ -38,7 +38,8 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=5)
     p.add_argument("--warmup", type=int, default=2)
-    p.add_argument("--config", default="c5")
+    p.add_argument("--config", default="c5", help="c5 (default) or c4 (16 uneven subdomains, 40 tracers)")
+    p.add_argument("--graph-steps", type=int, default=20, help="c4: timesteps captured in one hipGraph")
     p.add_argument("--bc-group", type=int, default=20, help="tracers per Bermejo-Conde group")
     p.add_argument("--fp64-fields", type=int, default=16, help="fields in the plain FP64 timing leg")
     p.add_argument("--err-fields", type=int, default=0, help="fields checked against FP64 (0 = all)")
@@ -65,6 +66,8 @@ def main():
 
     assert torch.cuda.is_available(), "bench.py needs a HIP device"
     dev = torch.device("cuda:0")
+    if args.config == "c4":
+        return run_c4(args, np, torch, L, A, S, dev)
     cfg = S.CONFIGS[args.config]
     g = S.grid_desc(cfg)
     nf = cfg.nfields
@@ -329,6 +332,153 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+
+
+def run_c4(args, np, torch, L, A, S, dev):
+    """Batching workload: 16 subdomains of uneven size (64x48x84 .. 512x384x84), each
+    with its own seeded winds and trajectory; 40 FP32 tracers per subdomain share its
+    thermodynamic departure points.  All tracers of all subdomains go through ONE
+    persistent launch per timestep (tile parameters from adz_plan_autotune); a hipGraph
+    holds `graph_steps` timesteps of (16 x adz_traject + the launch).  Plain per-field
+    FP64 launches (adz_tricub_hlt, one field per call) run beside it."""
+    subs = S.c4_subdomains()
+    ntr = subs[0].ntracers
+    t_setup = time.time()
+    state = []
+    for cfg in subs:
+        g = S.grid_desc(cfg)
+        ctx = A.AdzContext(g, L.ADZ_F64, itraj=cfg.niter)
+        w = {k: v.to(dev) for k, v in S.winds(cfg, g, device=dev).items()}
+        mod = (cfg.nk, cfg.nj + 2 * MH, cfg.ni + 2 * MH)
+        rng = np.random.default_rng(cfg.seed)
+        src = [S.lognormal_front_tracer(mod, rng, dev).reshape(-1).float() for _ in range(ntr)]
+        dst = [torch.zeros_like(x) for x in src]
+        i = ctx.info
+        box = (i["i0"], i["in"], i["j0"], i["jn"], i["k0t"], cfg.nk)
+        dl = (1 - MH, cfg.ni + MH, 1 - MH, cfg.nj + MH)
+        off = MH + (cfg.ni + 2 * MH) * MH
+        ctx.adz_inittraj()
+        state.append(dict(cfg=cfg, ctx=ctx, w=w, src=src, dst=dst, box=box, dl=dl, off=off, n_mod=math.prod(mod)))
+
+    def traj(st):
+        w = st["w"]
+        st["ctx"].adz_traject(st["cfg"].dt, w["uu_ext"], w["vv_ext"], w["ww_ext"], w["ut0"], w["vt0"], w["zdt0"])
+
+    for st in state:
+        traj(st)
+    items = []
+    for st in state:
+        stack = [dict(src=a, dst=b) for a, b in zip(st["src"], st["dst"])]
+        items.append(dict(geom=st["ctx"].ztab(True), xyz=st["ctx"].array(A.ADZ_A_PT), box=st["box"], dlayout=st["dl"],
+                          ext=False, fields=A.field_list(stack, st["off"], st["off"])))
+    npts = lambda b: (b[1] - b[0] + 1) * (b[3] - b[2] + 1) * (b[5] - b[4] + 1)
+    pts = sum(npts(st["box"]) for st in state)
+    pf = pts * ntr
+    log(f"c4: {len(state)} subdomains, {pts / 1e6:.1f} M target points, {pf / 1e9:.2f} G point-fields per step, "
+        f"setup {time.time() - t_setup:.1f}s")
+    t0 = time.time()
+    tune, tune_ms = A.Plan.autotune(items, L.ADZ_F32, math=L.ADZ_MATH_F32, reps=2)
+    log(f"autotune: {tune} -> {tune_ms:.2f} ms per launch ({time.time() - t0:.1f}s)")
+    plan = A.Plan(items, L.ADZ_F32, math=L.ADZ_MATH_F32, tune=tune)
+    default_plan = A.Plan(items, L.ADZ_F32, math=L.ADZ_MATH_F32)
+
+    def step():
+        for st in state:
+            traj(st)
+        plan.execute()
+
+    ev = lambda: torch.cuda.Event(enable_timing=True)
+
+    def timed(fn, reps):
+        torch.cuda.synchronize()
+        a, b = ev(), ev()
+        a.record()
+        for _ in range(reps):
+            fn()
+        b.record()
+        torch.cuda.synchronize()
+        return a.elapsed_time(b) / reps
+
+    for _ in range(args.warmup):
+        step()
+    nst = args.graph_steps
+    graph = torch.cuda.CUDAGraph()
+    cs = torch.cuda.Stream()
+    cs.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(cs):
+        step()
+        torch.cuda.synchronize()
+        with torch.cuda.graph(graph, stream=cs):
+            for _ in range(nst):
+                step()
+    torch.cuda.current_stream().wait_stream(cs)
+    graph.replay()
+    torch.cuda.synchronize()
+    reps = max(1, args.steps // nst)
+    t_graph = timed(graph.replay, reps)
+    ms_step = t_graph / nst
+    t_plan = timed(plan.execute, 5)
+    t_plan_default = timed(default_plan.execute, 5)
+    t_traj = timed(lambda: [traj(st) for st in state], 2)
+    log(f"step {ms_step:.2f} ms (graph of {nst} steps x {reps}); persistent launch {t_plan:.2f} ms "
+        f"(default tiles {t_plan_default:.2f} ms); 16 trajectories {t_traj:.2f} ms")
+
+    # plain per-field FP64 launches beside it, and the FP32 error against them
+    plan.execute()
+    torch.cuda.synchronize()
+    t64, worst_rms, worst_max, nerr = 0.0, 0.0, 0.0, 0
+    for n, st in enumerate(state):
+        nf64 = 4 if n % 5 else ntr  # every field on a few subdomains, 4 on the others (time scales per field)
+        for f in range(nf64):
+            s64 = st["src"][f].double()
+            d64 = torch.zeros_like(s64)
+            run = lambda: A.adz_tricub_hlt([dict(src=s64, dst=d64)], st["ctx"].array(A.ADZ_A_PT), st["ctx"].ztab(True),
+                                           *st["box"][:5], st["box"][5], st["dl"], ext=False, math=L.ADZ_MATH_REF,
+                                           src_off=st["off"])
+            run()
+            t64 += timed(run, 1) * (ntr / nf64)
+            i0, in_, j0, jn, k0, nk = st["box"]
+            cfg = st["cfg"]
+            mod = (cfg.nk, cfg.nj + 2 * MH, cfg.ni + 2 * MH)
+            sl = (slice(k0 - 1, None), slice(j0 - 1 + MH, jn + MH), slice(i0 - 1 + MH, in_ + MH))
+            a64 = d64.reshape(mod)[sl]
+            a32 = st["dst"][f].reshape(mod)[sl].double()
+            diff = a32 - a64
+            worst_rms = max(worst_rms, float(torch.linalg.vector_norm(diff) / torch.linalg.vector_norm(a64)))
+            worst_max = max(worst_max, float(diff.abs().max() / a64.abs().max()))
+            nerr += 1
+            del s64, d64
+    plain_gpts = pf / (t64 * 1e-3) / 1e9
+    log(f"plain FP64 per-field launches: {t64:.1f} ms per step ({plain_gpts:.2f} Gpt/s); FP32 vs FP64 worst "
+        f"RMS {worst_rms:.2e}, max {worst_max:.2e} over {nerr} fields")
+    alg = sum(ntr * 4 * (st["n_mod"] + npts(st["box"])) + 24 * npts(st["box"]) for st in state)
+    achieved = alg / (t_plan * 1e-3) / 1e9
+    out = {
+        "metric": "interpolated Gpoints/s + HBM GB/s: FP32 batched vs FP64 plain at 1e-5 RMS",
+        "value": round(pf / (ms_step * 1e-3) / 1e9, 3),
+        "unit": "Gpoints/s",
+        "n_gpus": 1, "steps": reps * nst, "warmup": args.warmup, "ms_per_step": round(ms_step, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded per-subdomain winds, lognormal front tracers, generated in HBM)",
+        "config": {"workload": f"c4: 16 subdomains 64x48x84..512x384x84, {ntr} FP32 tracers each, one persistent "
+                               f"launch per step, {nst} steps (trajectories + launch) per hipGraph",
+                   "points_x_fields_per_step": pf, "tune": tune},
+        "phases_ms": {"traject_16": round(t_traj, 3), "persistent_launch": round(t_plan, 3),
+                      "persistent_launch_default_tiles": round(t_plan_default, 3),
+                      "plain_fp64_per_field_launches": round(t64, 3)},
+        "interp_gpoints_per_s": round(pf / (t_plan * 1e-3) / 1e9, 3),
+        "fp64_plain_gpoints_per_s": round(plain_gpts, 3),
+        "speedup_batched_fp32_vs_fp64_plain": round(t64 / t_plan, 3),
+        "error_vs_fp64": {"worst_max_rel": worst_max, "worst_rms_rel": worst_rms, "fields": nerr},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK, 4), "traffic": None,
+                     "kernel": f"k_plan<float,double,F32,0> over 16 items, {t_plan:.3f} ms/launch, "
+                               f"{alg / 1e9:.2f} GB algorithmic"},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+    plan.free()
+    default_plan.free()
 
 
 def cpu_baseline(np, torch, cfg, g, src, pt, tbox, mod_shape, nf, seconds):

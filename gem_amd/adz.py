@@ -139,6 +139,7 @@ L.extra_sigs({
     "adz_plan_execute": ("i", ["vp", "vp"]),
     "adz_plan_tune": ("i", ["vp", "vp"]),
     "adz_plan_free": ("i", ["vp"]),
+    "adz_plan_autotune": ("i", ["vp", "i", "i", "i", "i", "i", "vp", "vp", "vp"]),
 })
 
 ADZ_POST_STORE, ADZ_POST_CLIP = 1, 2
@@ -158,7 +159,7 @@ class AdzItem(C.Structure):
 
 class AdzTune(C.Structure):
     _fields_ = [("fields_per_pass", C.c_int), ("points_outer", C.c_int), ("blocks_per_cu", C.c_int),
-                ("xcd_chunks", C.c_int)]
+                ("xcd_chunks", C.c_int), ("stage", C.c_int)]
 
 
 def _ptr_at(t, off):
@@ -195,32 +196,50 @@ def adz_tricub_hlt(stack, xyz, geom, i0, in_, j0, jn, k0, nk, dlayout, ext=True,
             "adz_tricub_hlt")
 
 
+def _item_array(items):
+    """adz_item array for a list of dicts {geom, xyz, box=(i0,in,j0,jn,k0,nk),
+    dlayout=(dminx,dmaxx,dminy,dmaxy), ext, fields=field_list(...)}; returns the array
+    and the field lists it points into (keep them alive while the array is used)."""
+    keep = []
+    arr = (AdzItem * len(items))()
+    for q, it in enumerate(items):
+        fl = it["fields"]
+        keep.append(fl)
+        a = arr[q]
+        a.geom = it["geom"]
+        a.xyz = it["xyz"].data_ptr()
+        a.pdtype = _DT[it["xyz"].dtype]
+        a.i0, a.in_, a.j0, a.jn, a.k0, a.nk = it["box"]
+        a.dminx, a.dmaxx, a.dminy, a.dmaxy = it["dlayout"]
+        a.ext = 1 if it.get("ext", True) else 0
+        a.fld = C.cast(fl, C.c_void_p)
+        a.nptr = len(fl)
+    return arr, keep
+
+
 class Plan:
     """A persistent batched launch over several subdomains (adz_plan_*)."""
 
     def __init__(self, items, fdtype, math=L.ADZ_MATH_F32, post=0, tune=None):
-        self._keep = []
-        arr = (AdzItem * len(items))()
-        for q, it in enumerate(items):
-            fl = it["fields"]
-            self._keep.append(fl)
-            a = arr[q]
-            a.geom = it["geom"]
-            a.xyz = it["xyz"].data_ptr()
-            a.pdtype = _DT[it["xyz"].dtype]
-            a.i0, a.in_, a.j0, a.jn, a.k0, a.nk = it["box"]
-            a.dminx, a.dmaxx, a.dminy, a.dmaxy = it["dlayout"]
-            a.ext = 1 if it.get("ext", True) else 0
-            a.fld = C.cast(fl, C.c_void_p)
-            a.nptr = len(fl)
+        arr, self._keep = _item_array(items)
         t = None
         if tune is not None:
-            t = AdzTune(*[int(tune.get(k, 0)) for k in ("fields_per_pass", "points_outer", "blocks_per_cu",
-                                                         "xcd_chunks")])
+            t = AdzTune(*[int(tune.get(k, 0)) for k, _ in AdzTune._fields_])
         h = C.c_void_p()
         L.check(L.lib().adz_plan_create(arr, len(items), fdtype, math, post, C.byref(t) if t else None,
                                         C.byref(h)), "adz_plan_create")
         self.h = h
+
+    @staticmethod
+    def autotune(items, fdtype, math=L.ADZ_MATH_F32, post=0, reps=3):
+        """adz_plan_autotune: the fastest tile parameters for these items (the items'
+        destinations are written).  Returns (tune dict, ms per execution)."""
+        arr, keep = _item_array(items)
+        t, ms = AdzTune(), C.c_float()
+        L.check(L.lib().adz_plan_autotune(arr, len(items), fdtype, math, post, reps, _stream(), C.byref(t),
+                                          C.byref(ms)), "adz_plan_autotune")
+        del keep
+        return {k: getattr(t, k) for k, _ in AdzTune._fields_}, ms.value
 
     def execute(self):
         L.check(L.lib().adz_plan_execute(self.h, _stream()), "adz_plan_execute")

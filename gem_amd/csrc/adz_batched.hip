@@ -11,6 +11,7 @@
 // XCDs), so each XCD's L2 sees spatially adjacent tiles.
 #include <algorithm>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "adz_common.h"
@@ -18,7 +19,12 @@
 
 namespace adz {
 
-constexpr int kTile = 256;
+constexpr int kTI = 64;       // targets per tile along i (one wavefront)
+constexpr int kTJ = 4;        // rows of targets per tile (four wavefronts)
+constexpr int kTile = kTI * kTJ;
+constexpr int kPitch = 128;       // LDS row pitch (elements): a box row spans at most 128 columns
+constexpr int kMaxRows = 39;      // box rows (y x planes) per LDS buffer; two buffers per block
+                                  // (2 x 39 x 128 x 4 B + tables: four FP32 blocks fit a CU's 160 KB)
 constexpr int kInlineFields = 16;
 
 struct DevItem {
@@ -42,6 +48,7 @@ struct PlanArgs {
   int g;
   int points_outer;
   int xcd_chunks;
+  int stage;
 };
 
 struct InlineFields {
@@ -52,81 +59,174 @@ struct InlineFields {
   void* mx[kInlineFields];
 };
 
-// One tile: target points [ptile*256, ptile*256+256) of `it`, fields [f0, f1).
-template <class TF, class TP, int MATH, int POST, class Fields>
-__device__ __forceinline__ void run_tile(const DevItem& it, const Fields& F, long ptile, int f0, int f1) {
-  long p = ptile * kTile + threadIdx.x;
-  if (p >= it.npts) return;
-  int i = it.i0 + (int)(p % it.nx);
-  long r = p / it.nx;
-  int j = it.j0 + (int)(r % it.ny);
-  int k = it.k0 + (int)(r / it.ny);
-  long o = (long)(i - 1) + (long)it.dni * ((j - 1) + (long)it.dnj * (k - 1));  // dst(i,j,k) from dst(1,1,1)
-  double px, py, pz;
-  load_pos<TP>(it.xyz, p, &px, &py, &pz);
-  Index x;
-  const bool ok = index_step(px, py, pz, it.lv, &x);
-  const long ni = it.lv.ni;
-  const long o1 = x.zlinear ? 0 : (long)it.lv.nij;
-  const long o2 = o1 + it.lv.nij, o3 = o2 + o1;
+// Evaluate one target from a source addressed with row stride `ni` and plane stride `nij`
+// (global memory or the LDS box) and store the outputs at dst offset o.
+template <class TF, int MATH, int POST, class Fields, class SrcT>
+__device__ __forceinline__ void eval_store(const Fields& F, int f, const SrcT* f1p, long ni, long nij, int zlinear,
+                                           const void* cw, long o) {
+  const long o1 = zlinear ? 0 : nij;
+  const long o2 = o1 + nij, o3 = o2 + o1;
+  TF* d = (TF*)F.dst[f];
   if (MATH == ADZ_MATH_REF) {
-    double c[20];
-    if (ok) weights_ref(x, it.lv, c);
-    for (int f = f0; f < f1; f++) {
-      TF* d = (TF*)F.dst[f];
-      if (!ok) {
-        TF nan = from_f64<TF>(__builtin_nan(""));
-        d[o] = nan;
-        if (POST & ADZ_POST_STORE) { ((TF*)F.lin[f])[o] = nan; ((TF*)F.mn[f])[o] = nan; ((TF*)F.mx[f])[o] = nan; }
-        continue;
+    const double* c = (const double*)cw;
+    double v = tricub_ref<SrcT>(f1p, c, ni, o1, o2, o3);
+    if (POST) {
+      double l, mi, ma;
+      mono_extra<SrcT, double>(f1p, c, ni, o1, o2, &l, &mi, &ma);
+      // adz_interp_hlt.F90:158-160 narrows each to real before clipping lin
+      TF ls = from_f64<TF>(l), mis = from_f64<TF>(mi), mas = from_f64<TF>(ma);
+      TF vs = from_f64<TF>(v);
+      if (POST & ADZ_POST_CLIP) vs = from_f64<TF>(fmax(to_f64(mis), fmin(to_f64(mas), to_f64(vs))));
+      d[o] = vs;
+      if (POST & ADZ_POST_STORE) {
+        ((TF*)F.lin[f])[o] = from_f64<TF>(fmax(to_f64(mis), fmin(to_f64(mas), to_f64(ls))));
+        ((TF*)F.mn[f])[o] = mis;
+        ((TF*)F.mx[f])[o] = mas;
       }
-      const TF* f1p = (const TF*)F.src[f] + x.off;
-      double v = tricub_ref<TF>(f1p, c, ni, o1, o2, o3);
-      if (POST) {
-        double l, mi, ma;
-        mono_extra<TF, double>(f1p, c, ni, o1, o2, &l, &mi, &ma);
-        // adz_interp_hlt.F90:158-160 narrows each to real before clipping lin
-        TF ls = from_f64<TF>(l), mis = from_f64<TF>(mi), mas = from_f64<TF>(ma);
-        TF vs = from_f64<TF>(v);
-        if (POST & ADZ_POST_CLIP) vs = from_f64<TF>(fmax(to_f64(mis), fmin(to_f64(mas), to_f64(vs))));
-        d[o] = vs;
-        if (POST & ADZ_POST_STORE) {
-          ((TF*)F.lin[f])[o] = from_f64<TF>(fmax(to_f64(mis), fmin(to_f64(mas), to_f64(ls))));
-          ((TF*)F.mn[f])[o] = mis;
-          ((TF*)F.mx[f])[o] = mas;
-        }
-      } else {
-        d[o] = from_f64<TF>(v);
-      }
+    } else {
+      d[o] = from_f64<TF>(v);
     }
   } else {
-    float c[20];
-    if (ok) weights_f32(x, it.lv, c);
-    for (int f = f0; f < f1; f++) {
-      TF* d = (TF*)F.dst[f];
-      if (!ok) {
-        TF nan = from_f32<TF>(__builtin_nanf(""));
-        d[o] = nan;
-        if (POST & ADZ_POST_STORE) { ((TF*)F.lin[f])[o] = nan; ((TF*)F.mn[f])[o] = nan; ((TF*)F.mx[f])[o] = nan; }
-        continue;
+    const float* c = (const float*)cw;
+    float v = tricub_f32<SrcT>(f1p, c, ni, o1, o2, o3);
+    if (POST) {
+      float l, mi, ma;
+      mono_extra<SrcT, float>(f1p, c, ni, o1, o2, &l, &mi, &ma);
+      TF ls = from_f32<TF>(l), mis = from_f32<TF>(mi), mas = from_f32<TF>(ma);
+      TF vs = from_f32<TF>(v);
+      if (POST & ADZ_POST_CLIP) vs = from_f32<TF>(fmaxf(to_f32(mis), fminf(to_f32(mas), to_f32(vs))));
+      d[o] = vs;
+      if (POST & ADZ_POST_STORE) {
+        ((TF*)F.lin[f])[o] = from_f32<TF>(fmaxf(to_f32(mis), fminf(to_f32(mas), to_f32(ls))));
+        ((TF*)F.mn[f])[o] = mis;
+        ((TF*)F.mx[f])[o] = mas;
       }
-      const TF* f1p = (const TF*)F.src[f] + x.off;
-      float v = tricub_f32<TF>(f1p, c, ni, o1, o2, o3);
-      if (POST) {
-        float l, mi, ma;
-        mono_extra<TF, float>(f1p, c, ni, o1, o2, &l, &mi, &ma);
-        TF ls = from_f32<TF>(l), mis = from_f32<TF>(mi), mas = from_f32<TF>(ma);
-        TF vs = from_f32<TF>(v);
-        if (POST & ADZ_POST_CLIP) vs = from_f32<TF>(fmaxf(to_f32(mis), fminf(to_f32(mas), to_f32(vs))));
-        d[o] = vs;
-        if (POST & ADZ_POST_STORE) {
-          ((TF*)F.lin[f])[o] = from_f32<TF>(fmaxf(to_f32(mis), fminf(to_f32(mas), to_f32(ls))));
-          ((TF*)F.mn[f])[o] = mis;
-          ((TF*)F.mx[f])[o] = mas;
+    } else {
+      d[o] = from_f32<TF>(v);
+    }
+  }
+}
+
+template <class TF, int MATH, int POST, class Fields>
+__device__ __forceinline__ void store_nan(const Fields& F, int f, long o) {
+  TF nan = (MATH == ADZ_MATH_REF) ? from_f64<TF>(__builtin_nan("")) : from_f32<TF>(__builtin_nanf(""));
+  ((TF*)F.dst[f])[o] = nan;
+  if (POST & ADZ_POST_STORE) { ((TF*)F.lin[f])[o] = nan; ((TF*)F.mn[f])[o] = nan; ((TF*)F.mx[f])[o] = nan; }
+}
+
+// block-wide minimum of 6 values over 256 lanes (4 waves of 64); `red` = 24 ints of LDS
+__device__ __forceinline__ void block_min6(int* v, int* red) {
+#pragma unroll
+  for (int q = 0; q < 6; q++)
+    for (int s = 32; s > 0; s >>= 1) v[q] = min(v[q], __shfl_xor(v[q], s, 64));
+  if ((threadIdx.x & 63) == 0)
+#pragma unroll
+    for (int q = 0; q < 6; q++) red[q * 4 + (threadIdx.x >> 6)] = v[q];
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < 6; q++) v[q] = min(min(red[q * 4], red[q * 4 + 1]), min(red[q * 4 + 2], red[q * 4 + 3]));
+  __syncthreads();
+}
+
+// One tile: TI x TJ targets (i fastest) of one level of item `it`, fields [f0, f1).
+// The union of the tile's stencils (its "box") is staged in LDS once per field with
+// coalesced loads, then every lane evaluates its stencil from LDS.  A box larger than
+// the LDS buffer (a tile straddling very different departure points) falls back to
+// per-lane global loads.  Both paths run the same arithmetic in the same order.
+template <class TF, class TP, int MATH, int POST, class Fields>
+__device__ __forceinline__ void run_tile(const DevItem& it, const Fields& F, long ptile, int f0, int f1, TF* boxA,
+                                         TF* boxB, int* red, int* rowoff, int stage) {
+  const int lane = threadIdx.x & (kTI - 1), wv = threadIdx.x / kTI;
+  const long nib = (it.nx + kTI - 1) / kTI, njb = (it.ny + kTJ - 1) / kTJ;
+  const int ib = (int)(ptile % nib);
+  const int jb = (int)((ptile / nib) % njb);
+  const int kk = (int)(ptile / (nib * njb));
+  const int ii = ib * kTI + lane, jj = jb * kTJ + wv;
+  const bool valid = ii < it.nx && jj < it.ny;
+  const int i = it.i0 + ii, j = it.j0 + jj, k = it.k0 + kk;
+  const long p = (long)ii + (long)it.nx * (jj + (long)it.ny * kk);
+  const long o = (long)(i - 1) + (long)it.dni * ((j - 1) + (long)it.dnj * (k - 1));  // dst(i,j,k) from dst(1,1,1)
+  Index x;
+  bool ok = false;
+  if (valid) {
+    double px, py, pz;
+    load_pos<TP>(it.xyz, p, &px, &py, &pz);
+    ok = index_step(px, py, pz, it.lv, &x);
+  }
+  double cw[20];  // weights (FP64, or FP32 in the first 10 slots' storage)
+  if (ok) {
+    if (MATH == ADZ_MATH_REF) weights_ref(x, it.lv, cw);
+    else weights_f32(x, it.lv, (float*)cw);
+  }
+  // box = union of the tile's stencils (columns ix-1..ix+2, rows iy-1..iy+2, planes pz..)
+  const int big = 1 << 30;
+  int ext[6] = {ok ? x.ix - 1 : big, ok ? -(x.ix + 2) : big, ok ? x.iy - 1 : big, ok ? -(x.iy + 2) : big,
+                ok ? x.pz : big, ok ? -(x.pz + (x.zlinear ? 1 : 3)) : big};
+  block_min6(ext, red);
+  const int x0 = ext[0], x1 = -ext[1], y0 = ext[2], y1 = -ext[3], z0 = ext[4], z1 = -ext[5];
+  if (x0 == big) {  // no target of the tile has an in-range stencil
+    if (valid)
+      for (int f = f0; f < f1; f++) store_nan<TF, MATH, POST>(F, f, o);
+    return;
+  }
+  const int nxb = x1 - x0 + 1, nyb = y1 - y0 + 1, nzb = z1 - z0 + 1;
+  const long vol = (long)nxb * nyb * nzb;
+  const long gni = it.lv.ni, gnij = it.lv.nij;
+  const int nrows = nyb * nzb;
+  if (stage != 2 && nrows <= kMaxRows && nxb <= kPitch) {
+    const long nijb = (long)kPitch * nyb;
+    const long boff = ok ? (long)(x.ix - 1 - x0) + (long)(x.iy - 1 - y0) * kPitch + (long)(x.pz - z0) * nijb : 0;
+    const long gbase = (long)(x0 - 1) + (long)(y0 - 1) * gni + (long)z0 * gnij;  // box corner from f(1,1,1)
+    if (threadIdx.x < nrows) rowoff[threadIdx.x] = (int)((threadIdx.x % nyb) * gni + (threadIdx.x / nyb) * gnij);
+    __syncthreads();
+    // Asynchronous staging: each wavefront copies whole box rows straight from global
+    // memory into LDS (global_load_lds, no registers), lanes along i.  Field f+1 is in
+    // flight while field f is evaluated from the other buffer.
+    constexpr int U = sizeof(TF) == 2 ? 2 : 4;          // bytes per lane per load
+    const int units = nxb * (int)sizeof(TF) / U;         // loads per row
+    auto issue = [&](int f, TF* buf) {
+      const char* src = (const char*)((const TF*)F.src[f] + gbase);
+      char* dst = (char*)buf;
+      for (int r = wv; r < nrows; r += kTJ) {
+        const char* grow = src + (long)rowoff[r] * sizeof(TF);
+        char* lrow = dst + (long)r * kPitch * sizeof(TF);
+        for (int c = 0; c < units; c += kTI) {
+          if (c + lane < units) {
+            const void* ga = grow + (long)(c + lane) * U;
+            void* la = lrow + (long)c * U;
+            if (U == 4)
+              __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)ga,
+                                               (__attribute__((address_space(3))) void*)la, 4, 0, 0);
+            else
+              __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)ga,
+                                               (__attribute__((address_space(3))) void*)la, 2, 0, 0);
+          }
         }
-      } else {
-        d[o] = from_f32<TF>(v);
       }
+    };
+    // Two distinct LDS buffers with static roles (the loop is unrolled by two) let the
+    // compiler prove the DMA target and the buffer being read are disjoint, so the reads
+    // of field f do not wait for the loads of field f+1 still in flight.
+    issue(f0, boxA);
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __syncthreads();
+    for (int f = f0; f < f1; f += 2) {
+      if (f + 1 < f1) issue(f + 1, boxB);
+      if (ok) eval_store<TF, MATH, POST, Fields, TF>(F, f, boxA + boff, kPitch, nijb, x.zlinear, cw, o);
+      else if (valid) store_nan<TF, MATH, POST>(F, f, o);
+      asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+      __syncthreads();
+      if (f + 1 >= f1) break;
+      if (f + 2 < f1) issue(f + 2, boxA);
+      if (ok) eval_store<TF, MATH, POST, Fields, TF>(F, f + 1, boxB + boff, kPitch, nijb, x.zlinear, cw, o);
+      else if (valid) store_nan<TF, MATH, POST>(F, f + 1, o);
+      asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+      __syncthreads();
+    }
+  } else {
+    for (int f = f0; f < f1; f++) {
+      if (ok) eval_store<TF, MATH, POST, Fields, TF>(F, f, (const TF*)F.src[f] + x.off, gni, gnij, x.zlinear, cw, o);
+      else if (valid) store_nan<TF, MATH, POST>(F, f, o);
     }
   }
 }
@@ -142,6 +242,10 @@ struct ItemFields {  // field pointer tables of a plan item (device arrays)
 // Persistent kernel over the tile list of a plan.
 template <class TF, class TP, int MATH, int POST>
 __global__ __launch_bounds__(kTile) void k_plan(PlanArgs A) {
+  __shared__ TF boxA[kMaxRows * kPitch];
+  __shared__ TF boxB[kMaxRows * kPitch];
+  __shared__ int red[24];
+  __shared__ int rowoff[kMaxRows];
   long lo = 0, hi = A.ntiles, step = gridDim.x, t0 = blockIdx.x;
   if (A.xcd_chunks) {
     int nb = gridDim.x / 8;                 // blocks per XCD (grid is a multiple of 8)
@@ -161,14 +265,18 @@ __global__ __launch_bounds__(kTile) void k_plan(PlanArgs A) {
     else { grp = local / it.ptiles; ptile = local % it.ptiles; }
     int f0 = (int)grp * A.g, f1 = min(it.nf, f0 + A.g);
     ItemFields F{it.src, it.dst, it.lin, it.mn, it.mx};
-    run_tile<TF, TP, MATH, POST, ItemFields>(it, F, ptile, f0, f1);
+    run_tile<TF, TP, MATH, POST, ItemFields>(it, F, ptile, f0, f1, boxA, boxB, red, rowoff, A.stage);
   }
 }
 
 // One item with inline field pointers (the adz_tricub_hlt entry point).
 template <class TF, class TP, int MATH, int POST>
 __global__ __launch_bounds__(kTile) void k_hlt(DevItem it, InlineFields F) {
-  run_tile<TF, TP, MATH, POST, InlineFields>(it, F, blockIdx.x, 0, it.nf);
+  __shared__ TF boxA[kMaxRows * kPitch];
+  __shared__ TF boxB[kMaxRows * kPitch];
+  __shared__ int red[24];
+  __shared__ int rowoff[kMaxRows];
+  run_tile<TF, TP, MATH, POST, InlineFields>(it, F, blockIdx.x, 0, it.nf, boxA, boxB, red, rowoff, 2);
 }
 
 }  // namespace adz
@@ -203,7 +311,7 @@ bool fill_item(DevItem* d, const adz_ztab* geom, const void* xyz, int i0, int in
   d->dni = dmaxx - dminx + 1; d->dnj = dmaxy - dminy + 1;
   d->nf = nf;
   d->npts = (long)d->nx * d->ny * d->nz;
-  d->ptiles = (d->npts + kTile - 1) / kTile;
+  d->ptiles = (long)((d->nx + kTI - 1) / kTI) * ((d->ny + kTJ - 1) / kTJ) * d->nz;
   return true;
 }
 
@@ -298,12 +406,13 @@ int adz_plan_create(const adz_item* items, int nitems, int fdtype, int math, int
   for (int q = 0; q < nitems; q++)
     if (items[q].pdtype != pdt) return fail(ADZ_EINVAL, "adz_plan_create: items must share the position dtype");
   if (!pick<true>(fdtype, pdt, math, post)) return fail(ADZ_EINVAL, "adz_plan_create: unsupported dtype/math");
-  adz_tune T{8, 0, 8, 1};
+  adz_tune T{8, 0, 4, 1, 2};
   if (tune) {
     if (tune->fields_per_pass > 0) T.fields_per_pass = std::min(64, tune->fields_per_pass);
     T.points_outer = tune->points_outer;
     if (tune->blocks_per_cu > 0) T.blocks_per_cu = tune->blocks_per_cu;
     T.xcd_chunks = tune->xcd_chunks;
+    if (tune->stage > 0) T.stage = tune->stage;
   }
   std::vector<DevItem> dv(nitems);
   long nptr_total = 0;
@@ -354,7 +463,7 @@ int adz_plan_create(const adz_item* items, int nitems, int fdtype, int math, int
   }
   adz_plan* p = new adz_plan;
   p->fdtype = fdtype; p->pdtype = pdt; p->math = math; p->post = post; p->tune = T; p->dmem = mem;
-  p->args = PlanArgs{(const DevItem*)mem, nitems, tiles, T.fields_per_pass, T.points_outer, T.xcd_chunks};
+  p->args = PlanArgs{(const DevItem*)mem, nitems, tiles, T.fields_per_pass, T.points_outer, T.xcd_chunks, T.stage};
   long grid = (long)T.blocks_per_cu * device_cus();
   if (grid > tiles) grid = tiles;
   if (T.xcd_chunks) grid = std::max(8L, (grid / 8) * 8);
@@ -375,6 +484,63 @@ int adz_plan_execute(const adz_plan* p, adz_stream stream) {
 int adz_plan_tune(const adz_plan* p, adz_tune* out) {
   if (!p || !out) return fail(ADZ_EINVAL, "adz_plan_tune: null argument");
   *out = p->tune;
+  return ADZ_OK;
+}
+
+// Autotuner: a coordinate search over the tile parameters, each candidate timed with
+// HIP events on the caller's plan (its destinations are written, with the values any
+// execution writes).  Order: staging, fields per pass, persistent grid size, loop order,
+// XCD chunking; each coordinate keeps the best value found before moving on.
+int adz_plan_autotune(const adz_item* items, int nitems, int fdtype, int math, int post, int reps,
+                      adz_stream stream, adz_tune* best, float* best_ms) {
+  if (!best) return fail(ADZ_EINVAL, "adz_plan_autotune: null result");
+  if (reps <= 0) reps = 3;
+  int maxf = 1;
+  for (int q = 0; q < nitems && items; q++) maxf = std::max(maxf, items[q].nptr);
+  hipStream_t s = (hipStream_t)stream;
+  hipEvent_t e0, e1;
+  if (hipEventCreate(&e0) != hipSuccess) return fail(ADZ_EHIP, "adz_plan_autotune: event");
+  if (hipEventCreate(&e1) != hipSuccess) { (void)hipEventDestroy(e0); return fail(ADZ_EHIP, "adz_plan_autotune: event"); }
+  auto time_one = [&](const adz_tune& t, float* ms) -> int {
+    adz_plan* p = nullptr;
+    int rc = adz_plan_create(items, nitems, fdtype, math, post, &t, &p);
+    if (rc != ADZ_OK) return rc;
+    rc = adz_plan_execute(p, stream);  // warm-up
+    if (rc == ADZ_OK && hipEventRecord(e0, s) != hipSuccess) rc = fail(ADZ_EHIP, "adz_plan_autotune: record");
+    for (int r = 0; r < reps && rc == ADZ_OK; r++) rc = adz_plan_execute(p, stream);
+    if (rc == ADZ_OK && (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                         hipEventElapsedTime(ms, e0, e1) != hipSuccess))
+      rc = fail(ADZ_EHIP, "adz_plan_autotune: timing");
+    if (rc == ADZ_OK) *ms /= reps;
+    int rf = adz_plan_free(p);
+    return rc != ADZ_OK ? rc : rf;
+  };
+  adz_tune cur{8, 0, 4, 1, 2};
+  cur.fields_per_pass = std::min(cur.fields_per_pass, maxf);
+  float cur_ms = 0.f;
+  int rc = time_one(cur, &cur_ms);
+  const int stages[] = {1, 2}, fpps[] = {2, 4, 8, 16, 32, 64}, bpcs[] = {2, 4, 8}, flags[] = {0, 1};
+  for (int coord = 0; coord < 5 && rc == ADZ_OK; coord++) {
+    const int* vals = coord == 0 ? stages : coord == 1 ? fpps : coord == 2 ? bpcs : flags;
+    int nv = coord == 0 ? 2 : coord == 1 ? 6 : coord == 2 ? 3 : 2;
+    for (int v = 0; v < nv && rc == ADZ_OK; v++) {
+      adz_tune t = cur;
+      int* slot = coord == 0 ? &t.stage : coord == 1 ? &t.fields_per_pass : coord == 2 ? &t.blocks_per_cu
+                : coord == 3 ? &t.points_outer : &t.xcd_chunks;
+      if (*slot == vals[v]) continue;
+      if (coord == 1 && vals[v] > maxf && vals[v - 1] >= maxf) continue;  // one candidate covers all fields
+      *slot = coord == 1 ? std::min(vals[v], maxf) : vals[v];
+      if (coord == 1 && t.fields_per_pass == cur.fields_per_pass) continue;
+      float ms = 0.f;
+      rc = time_one(t, &ms);
+      if (rc == ADZ_OK && ms < cur_ms) { cur = t; cur_ms = ms; }
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc != ADZ_OK) return rc;
+  *best = cur;
+  if (best_ms) *best_ms = cur_ms;
   return ADZ_OK;
 }
 

@@ -70,6 +70,8 @@ struct Index {
   long off;      // element offset of the stencil's lower-left corner from f(1,1,1)
   int zlinear;   // nonzero: linear along z (4x4x2 stencil)
   int iz0;       // iz - 1 (0-based level of the interval's lower level)
+  int ix, iy;    // local (1-based) index of the target cell; stencil columns ix-1..ix+2
+  int pz;        // 0-based first plane of the stencil (iz0 - 1 cubic, iz0 linear)
   double dx, dy, dz;
 };
 
@@ -92,6 +94,7 @@ __device__ __forceinline__ bool index_step(double px, double py, double pz, cons
   int zlinear = ((iz - 1) | (lv.nk - 3 - iz)) >> 31;
   if (!zlinear) ijk -= lv.nij;
   r->off = ijk; r->zlinear = zlinear; r->iz0 = iz; r->dx = dx; r->dy = dy; r->dz = dz;
+  r->ix = ix; r->iy = iy; r->pz = zlinear ? iz : iz - 1;
   return true;
 }
 

@@ -201,13 +201,15 @@ typedef struct adz_item {
   int nptr;
 } adz_item;
 
-/* Tile parameters of the persistent kernel; zero fields take the autotuned default. */
+/* Tile parameters of the persistent kernel; zero fields take the built-in default
+   {8, 0, 4, 1, 2}; adz_plan_autotune searches them on the actual plan. */
 typedef struct adz_tune {
-  int fields_per_pass; /* fields interpolated per pass over the points (1..16)          */
+  int fields_per_pass; /* fields interpolated per pass over the points (1..64)          */
   int points_outer;    /* 1: a block runs every field group of a point tile; 0: a pass
                           over all points per field group                               */
   int blocks_per_cu;   /* persistent grid = blocks_per_cu * CUs                         */
   int xcd_chunks;      /* 1: each XCD takes a contiguous range of tiles                 */
+  int stage;           /* 1: stage each tile's source box in LDS; 2: per-lane loads     */
 } adz_tune;
 
 typedef struct adz_plan adz_plan;
@@ -219,6 +221,13 @@ int adz_plan_create(const adz_item* items, int nitems, int fdtype, int math, int
 int adz_plan_execute(const adz_plan* p, adz_stream stream);
 int adz_plan_tune(const adz_plan* p, adz_tune* out); /* the parameters in use */
 int adz_plan_free(adz_plan* p);
+/* Project autotuner for the persistent kernel's tile parameters: a coordinate search
+ * (staging, fields per pass, blocks per CU, loop order, XCD chunking), each candidate
+ * timed over `reps` executions of a plan built from `items`; the items' destinations are
+ * written.  Returns the fastest tune and its time per execution (ms).  No counterpart in
+ * the reference (its slices are fixed at ni*nj points, adz_interp_hlt.F90:102). */
+int adz_plan_autotune(const adz_item* items, int nitems, int fdtype, int math, int post, int reps,
+                      adz_stream stream, adz_tune* best, float* best_ms);
 
 /* ------------------------------------------------------------------------------------
  * Level 4: Bermejo-Conde mass fixer (Bermejo_Conde.F90:18-320 with the sums of
