@@ -1,0 +1,23 @@
+"""CPU: the Fortran ISO_C_BINDING layer (gem_amd/fortran) builds with amdflang and its
+bind(C) derived types have the byte sizes of the C structs of include/adz_hip.h (a
+field missing, misplaced or of the wrong kind changes the size or the padding)."""
+import ctypes as C
+import os
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DRIVER = os.path.join(ROOT, "gem_amd", "fortran", "adz_f_driver")
+
+
+def test_bind_c_type_sizes_match_c_abi():
+    if not os.path.exists(DRIVER):
+        pytest.skip("Fortran driver not built (no amdflang)")
+    from gem_amd import adz as A
+
+    out = subprocess.run([DRIVER, "sizes"], capture_output=True, text=True, timeout=60)
+    line = [ln for ln in out.stdout.splitlines() if ln.startswith("sizes")][0]
+    sizes = [int(x) for x in line.split()[1:]]
+    assert sizes == [C.sizeof(s) for s in (A.AdzGrid, A.AdzField, A.AdzItem, A.AdzTune, A.AdzBcTracer,
+                                           A.AdzBcDomain)]
