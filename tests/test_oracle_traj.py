@@ -142,3 +142,48 @@ def test_traject_deterministic_and_storage_consistent(grid):
     assert all(np.array_equal(a[k], b[k]) for k in a)
     c = G.traject(w, cfg.dt, storage="f64")
     assert np.abs(a["pt"].astype(np.float64) - c["pt"]).max() < 1e-3
+
+
+def test_quadratic_wind_closed_form():
+    """u = U(x) cos(lat), U quadratic in the grid index x, v = w = 0.  The midpoint
+    iteration's fixed point solves x_D + k U(x_D) = x_A - k U(x_A), k = dt/2 /(a h), and
+    the cubic interpolation of the departure wind is exact for a quadratic, so every
+    departure point has a closed form.  A quadratic (unlike a constant or linear wind)
+    also pins the 4-point destaggering of the arrival winds ut0 (a symmetric stencil with
+    swapped inner/outer weights reproduces linears, not quadratics), the 1/cos(lat)
+    factor and the staggered u-grid midpoints."""
+    cfg = small_cfg()
+    g = S.grid_desc(cfg)
+    G = O.Grid(g)
+    i = G.info
+    nit, njt, nk = cfg.ni + 2 * cfg.halo, cfg.nj + 2 * cfg.halo, cfg.nk
+    xc, u0, u1, u2 = cfg.ni / 2.0, 20.0, 0.2, 0.004
+    U = lambda x: u0 + u1 * (x - xc) + u2 * (x - xc) ** 2
+    xs = np.arange(i["lminx"], i["lmaxx"] + 1, dtype=np.float64)
+    cosl = np.cos(np.asarray(g["lat"]))[None, :, None]
+    shape = (nk, njt, nit)
+    uu = np.broadcast_to(U(xs)[None, None, :] * cosl, shape).astype(np.float32).reshape(-1)
+    ut = np.broadcast_to(U(xs + 0.5)[None, None, :] * cosl, shape).astype(np.float32).reshape(-1)
+    z = np.zeros(uu.size, np.float32)
+    r = G.traject(dict(uu_ext=uu, vv_ext=z, ww_ext=z, ut0=ut, vt0=z, zdt0=z), cfg.dt, storage="f64")
+    kap = 0.5 * cfg.dt * g["inv_rayt"] * g["inv_hx"]
+
+    def x_dep(xa):  # Newton on s + kap U(s) = xa - kap U(xa)
+        rhs = xa - kap * U(xa)
+        s = xa.copy()
+        for _ in range(40):
+            s = s - (s + kap * U(s) - rhs) / (1.0 + kap * (u1 + 2 * u2 * (s - xc)))
+        return s
+
+    ny, nx = i["jn"] - i["j0"] + 1, i["in"] - i["i0"] + 1
+    pm = r["pm"].reshape(nk, ny, nx, 3)
+    xe = x_dep(np.arange(i["i0"], i["in"] + 1, dtype=np.float64))
+    inside = (xe > i["iminposx"] + 0.1) & (xe < i["imaxposx"] - 0.1)
+    assert inside.sum() > nx // 2
+    assert np.abs(pm[..., 0] - xe[None, None, :])[..., inside].max() < 1e-6
+    assert np.abs(pm[..., 1] - np.arange(i["j0"], i["jn"] + 1)[None, :, None]).max() < 1e-9
+    nxu = i["inu"] - i["i0u"] + 1
+    pmu = r["pmu"].reshape(nk, ny, nxu, 3)
+    xu = x_dep(np.arange(i["i0u"], i["inu"] + 1, dtype=np.float64) + 0.5) - 0.5
+    inu = (xu > i["iminposx"] + 0.1) & (xu < i["imaxposx"] - 0.1)
+    assert np.abs(pmu[..., 0] - xu[None, None, :])[..., inu].max() < 1e-6
