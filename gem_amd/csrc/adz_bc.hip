@@ -43,7 +43,7 @@ struct BcArgs {
   long npts;
   int min_max;
   int nblk;                  // blocks per tracer
-  long chunk;                // elements per block
+  int nrows;                 // scope rows (j fastest, then k)
   const void* airm1;
   const void* airm0;
   const void* pkps;
@@ -111,25 +111,25 @@ __device__ __forceinline__ double mass_term(T x, T air, double mask) {
   return (double)prod * mask;
 }
 
+// Grid: x = tracer (fastest), y = block of scope rows, so the blocks of every tracer over
+// the same rows are dispatched together and share the air mass, mask and pkps through
+// the caches.  Each block owns whole rows (a fixed partition), lanes stride along i.
 template <class T, int PASS>
 __global__ __launch_bounds__(kBcBlock) void k_bc_pass(BcArgs A) {
-  const int n = blockIdx.y;
+  const int n = blockIdx.x;
   const BcTr t = A.tr[n];
   const double* sc = A.sc + n * kSc;
   const bool cyc = (PASS >= P_CORR) && sc[SC_CYCLE] != 0.0;
   const double sd = sgn1(sc[SC_D]);
   const double lambda = sc[SC_LAMBDA], prop = sc[SC_PROP];
   DD s0{0.0, 0.0}, s1{0.0, 0.0};
-  long e0 = (long)blockIdx.x * A.chunk, e1 = e0 + A.chunk;
-  if (e1 > A.npts) e1 = A.npts;
+  const int r0 = (int)((long)A.nrows * blockIdx.y / A.nblk), r1 = (int)((long)A.nrows * (blockIdx.y + 1) / A.nblk);
   const T* air1 = (const T*)A.airm1;
   const T* air0 = (const T*)A.airm0;
-  for (long e = e0 + threadIdx.x; e < e1; e += kBcBlock) {
-    int i = A.i0 + (int)(e % A.nx);
-    long r = e / A.nx;
-    int j = A.j0 + (int)(r % A.ny);
-    int k = A.k0 + (int)(r / A.ny);
-    long q2 = (long)(i - 1) + (long)A.nxa * (j - 1);
+  for (int r = r0; r < r1; r++)
+  for (int ii = threadIdx.x; ii < A.nx; ii += kBcBlock) {
+    const int j = A.j0 + r % A.ny, k = A.k0 + r / A.ny;
+    long q2 = (long)(A.i0 - 1 + ii) + (long)A.nxa * (j - 1);
     long q = q2 + (long)A.nxa * A.nya * (k - 1);
     double mk = A.mask[q2];
     T* mp = (T*)t.m + q;
@@ -183,7 +183,7 @@ __global__ __launch_bounds__(kBcBlock) void k_bc_pass(BcArgs A) {
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    double* o = A.part + ((long)n * A.nblk + blockIdx.x) * 4;
+    double* o = A.part + ((long)n * A.nblk + blockIdx.y) * 4;
     o[0] = sh[0][0]; o[1] = sh[1][0]; o[2] = sh[2][0]; o[3] = sh[3][0];
   }
 }
@@ -264,12 +264,13 @@ int adz_bc_create(const adz_bc_tracer* tr, int ntr, const adz_bc_domain* dom, in
   a.i0 = d.i0; a.j0 = d.j0; a.k0 = d.k0; a.nx = d.in - d.i0 + 1; a.ny = d.jn - d.j0 + 1;
   a.npts = (long)a.nx * a.ny * (d.nk - d.k0 + 1);
   a.min_max = d.min_max;
+  a.nrows = a.ny * (d.nk - d.k0 + 1);
   long per = (long)kBcBlock * 16;
   long nblk = (a.npts + per - 1) / per;
   if (nblk > 2048) nblk = 2048;
+  if (nblk > a.nrows) nblk = a.nrows;
   if (nblk < 1) nblk = 1;
   a.nblk = (int)nblk;
-  a.chunk = (a.npts + nblk - 1) / nblk;
   size_t tb = sizeof(BcTr) * ntr, pb = sizeof(double) * 4 * ntr * nblk, sb = sizeof(double) * kSc * ntr;
   void* mem = nullptr;
   if (hipMalloc(&mem, tb + pb + sb) != hipSuccess) return fail(ADZ_ENOMEM, "adz_bc_create: hipMalloc");
@@ -296,7 +297,7 @@ int adz_bermejo_conde(adz_bc* bc, const void* airm1, const void* airm0, const vo
   a.airm1 = airm1; a.airm0 = airm0; a.pkps = pkps; a.mask = mask; a.deficit = deficit; a.diag = diag;
   if (bc->has_w3 && !pkps) return fail(ADZ_EINVAL, "adz_bermejo_conde: weight 3 needs pkps");
   hipStream_t st = (hipStream_t)stream;
-  dim3 g((unsigned)a.nblk, (unsigned)a.ntr), b(kBcBlock);
+  dim3 g((unsigned)a.ntr, (unsigned)a.nblk), b(kBcBlock);
   dim3 gf((unsigned)((a.ntr + 63) / 64)), bf(64);
 #define ADZ_BC_PASSES(T)                                                       \
   hipLaunchKernelGGL((k_bc_pass<T, P_PM>), g, b, 0, st, a);                   \
