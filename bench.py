@@ -140,7 +140,9 @@ def main():
     itemsA = [item("u", [dict(src=src[0], dst=dst[0])], True), item("v", [dict(src=src[1], dst=dst[1])], True),
               item("t", [dict(src=src[2], dst=dst[2])], True),
               item("t", [dict(src=src[3], dst=dst[3])], False)]
-    planA = A.Plan(itemsA, L.ADZ_F32, math=L.ADZ_MATH_F32, post=0)
+    t_tune = time.time()
+    tuneA, _ = A.Plan.autotune(itemsA, L.ADZ_F32, math=L.ADZ_MATH_F32, post=0, reps=2)
+    planA = A.Plan(itemsA, L.ADZ_F32, math=L.ADZ_MATH_F32, post=0, tune=tuneA)
     # plans B: tracer groups with lin/min/max for the fixer, and the fixer itself
     groups = []
     tbox = boxes["t"][0]
@@ -150,11 +152,16 @@ def main():
         ids = list(range(g0, min(nf, g0 + nb)))
         stack = [dict(src=src[q], dst=dst[q], lin=post[k]["lin"], min=post[k]["min"], max=post[k]["max"])
                  for k, q in enumerate(ids)]
-        plan = A.Plan([item("t", stack, False)], L.ADZ_F32, math=L.ADZ_MATH_F32, post=A.ADZ_POST_STORE)
+        itemsB = [item("t", stack, False)]
+        if not groups:  # the groups share one shape: tune the first, reuse its tiles
+            tuneB, _ = A.Plan.autotune(itemsB, L.ADZ_F32, math=L.ADZ_MATH_F32, post=A.ADZ_POST_STORE, reps=2)
+        plan = A.Plan(itemsB, L.ADZ_F32, math=L.ADZ_MATH_F32, post=A.ADZ_POST_STORE, tune=tuneB)
         bcs = [dict(p=src[q], m=dst[q], lin=post[k]["lin"], min=post[k]["min"], max=post[k]["max"], mass=1)
                for k, q in enumerate(ids)]
         bc = A.BermejoConde(bcs, layout5, scope, mod_off, min_max=True)
         groups.append((ids, plan, bc))
+    tuneB = tuneB if groups else None
+    log(f"autotuned tiles ({time.time() - t_tune:.1f}s): u/v/T/q plan {tuneA}, tracer groups {tuneB}")
     pts_total = npts(boxes["u"][0]) + npts(boxes["v"][0]) + (nf - 2) * npts(tbox)
     log(f"setup {time.time() - t_setup:.1f}s; {pts_total / 1e9:.3f} G interpolated point-fields per step")
 
@@ -316,7 +323,8 @@ def main():
                                f"{nf} fields (u, v, T, q + {ntr} tracers), FP32 storage, FP64 positions, "
                                f"Bermejo-Conde on {ntr} tracers, one step = trajectory + interpolation + fixer "
                                f"in one hipGraph",
-                   "points_x_fields_per_step": pts_total, "bc_group": nb},
+                   "points_x_fields_per_step": pts_total, "bc_group": nb,
+                   "tiles": {"uvtq": tuneA, "tracer_groups": tuneB}},
         "phases_ms": {"traject": round(t_traj, 3), "interpolation": round(t_interp, 3),
                       "bermejo_conde": round(t_bc, 3), "eager_step": round(eager_ms, 3)},
         "interp_gpoints_per_s": round(interp_gpts, 3),

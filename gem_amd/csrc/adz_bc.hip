@@ -126,42 +126,66 @@ __global__ __launch_bounds__(kBcBlock) void k_bc_pass(BcArgs A) {
   const int r0 = (int)((long)A.nrows * blockIdx.y / A.nblk), r1 = (int)((long)A.nrows * (blockIdx.y + 1) / A.nblk);
   const T* air1 = (const T*)A.airm1;
   const T* air0 = (const T*)A.airm0;
-  for (int r = r0; r < r1; r++)
-  for (int ii = threadIdx.x; ii < A.nx; ii += kBcBlock) {
+  // Each lane gathers the inputs of kU elements (kU * 256 apart along the row) before it
+  // accumulates them in order, so a wave keeps kU loads per array in flight.
+  constexpr int kU = 4;
+  const T* lin = (const T*)t.lin;
+  const T* pk = (const T*)A.pkps;
+  for (int r = r0; r < r1; r++) {
     const int j = A.j0 + r % A.ny, k = A.k0 + r / A.ny;
-    long q2 = (long)(A.i0 - 1 + ii) + (long)A.nxa * (j - 1);
-    long q = q2 + (long)A.nxa * A.nya * (k - 1);
-    double mk = A.mask[q2];
-    T* mp = (T*)t.m + q;
-    T m = *mp;
-    if (PASS == P_PM) {
-      dd_add(&s0, mass_term<T>(((const T*)t.p)[q], air1[q], mk));
-      dd_add(&s1, mass_term<T>(m, air0[q], mk));
-    } else if (PASS == P_W) {
-      double pk = (t.weight == 3) ? (double)((const T*)A.pkps)[q] : 0.0;
-      T w = bc_weight<T>(m, ((const T*)t.lin)[q], pk, t, sd);
-      dd_add(&s0, mass_term<T>(w, air0[q], mk));
-    } else if (PASS == P_CORR) {
-      if (!cyc) {
-        double pk = (t.weight == 3) ? (double)((const T*)A.pkps)[q] : 0.0;
-        T w = bc_weight<T>(m, ((const T*)t.lin)[q], pk, t, sd);
-        double corr = lambda * (double)w;
-        T bc = (T)((double)m - corr);
-        if (A.min_max) {
-          T mn = ((const T*)t.mn)[q], mx = ((const T*)t.mx)[q];
-          if (corr > 0.0 && bc < mn) bc = mn;
-          if (corr < 0.0 && bc > mx) bc = mx;
+    const long q2r = (long)(A.i0 - 1) + (long)A.nxa * (j - 1);
+    const long qr = q2r + (long)A.nxa * A.nya * (k - 1);
+    for (int ib = 0; ib < A.nx; ib += kU * kBcBlock) {
+      T mv[kU], av[kU], bv[kU], lv[kU], nv[kU], xv[kU];
+      double mk[kU], pkv[kU];
+#pragma unroll
+      for (int u = 0; u < kU; u++) {
+        const int ii = ib + u * kBcBlock + (int)threadIdx.x;
+        const bool in = ii < A.nx;
+        const long q = qr + (in ? ii : 0), q2 = q2r + (in ? ii : 0);
+        mk[u] = in ? A.mask[q2] : 0.0;
+        mv[u] = ((T*)t.m)[q];
+        av[u] = air0[q];
+        if (PASS == P_PM) { bv[u] = air1[q]; lv[u] = ((const T*)t.p)[q]; }
+        if (PASS == P_W || (PASS == P_CORR && !cyc)) {
+          lv[u] = lin[q];
+          pkv[u] = (t.weight == 3) ? (double)pk[q] : 0.0;
         }
-        m = bc;
-        *mp = m;
+        if (PASS == P_CORR && !cyc && A.min_max) { nv[u] = ((const T*)t.mn)[q]; xv[u] = ((const T*)t.mx)[q]; }
       }
-      dd_add(&s0, mass_term<T>(m, air0[q], mk));
-    } else {  // P_PROP
-      if (!cyc) {
-        m = (T)(prop * (double)m);
-        *mp = m;
+#pragma unroll
+      for (int u = 0; u < kU; u++) {
+        const int ii = ib + u * kBcBlock + (int)threadIdx.x;
+        if (ii >= A.nx) break;
+        T* mp = (T*)t.m + qr + ii;
+        T m = mv[u];
+        if (PASS == P_PM) {
+          dd_add(&s0, mass_term<T>(lv[u], bv[u], mk[u]));
+          dd_add(&s1, mass_term<T>(m, av[u], mk[u]));
+        } else if (PASS == P_W) {
+          T w = bc_weight<T>(m, lv[u], pkv[u], t, sd);
+          dd_add(&s0, mass_term<T>(w, av[u], mk[u]));
+        } else if (PASS == P_CORR) {
+          if (!cyc) {
+            T w = bc_weight<T>(m, lv[u], pkv[u], t, sd);
+            double corr = lambda * (double)w;
+            T bc = (T)((double)m - corr);
+            if (A.min_max) {
+              if (corr > 0.0 && bc < nv[u]) bc = nv[u];
+              if (corr < 0.0 && bc > xv[u]) bc = xv[u];
+            }
+            m = bc;
+            *mp = m;
+          }
+          dd_add(&s0, mass_term<T>(m, av[u], mk[u]));
+        } else {  // P_PROP
+          if (!cyc) {
+            m = (T)(prop * (double)m);
+            *mp = m;
+          }
+          dd_add(&s0, mass_term<T>(m, av[u], mk[u]));
+        }
       }
-      dd_add(&s0, mass_term<T>(m, air0[q], mk));
     }
   }
   // fixed-order block tree in LDS
