@@ -47,6 +47,9 @@ def parse():
     p.add_argument("--cpu-seconds", type=float, default=12.0)
     p.add_argument("--profile-kernel", action="store_true",
                    help="only build the workload and run the dominant interpolation kernel 5 times (rocprofv3)")
+    p.add_argument("--tiles", default='{"fields_per_pass": 20, "points_outer": 1, "blocks_per_cu": 8, '
+                                      '"xcd_chunks": 1, "stage": 2}',
+                   help="tracer-group tiles used by --profile-kernel (JSON; the timed run autotunes)")
     return p.parse_args()
 
 
@@ -141,7 +144,8 @@ def main():
               item("t", [dict(src=src[2], dst=dst[2])], True),
               item("t", [dict(src=src[3], dst=dst[3])], False)]
     t_tune = time.time()
-    tuneA, _ = A.Plan.autotune(itemsA, L.ADZ_F32, math=L.ADZ_MATH_F32, post=0, reps=2)
+    fixed = json.loads(args.tiles) if args.profile_kernel else None  # no tuning runs inside a profile
+    tuneA = fixed if fixed else A.Plan.autotune(itemsA, L.ADZ_F32, math=L.ADZ_MATH_F32, post=0, reps=2)[0]
     planA = A.Plan(itemsA, L.ADZ_F32, math=L.ADZ_MATH_F32, post=0, tune=tuneA)
     # plans B: tracer groups with lin/min/max for the fixer, and the fixer itself
     groups = []
@@ -154,7 +158,8 @@ def main():
                  for k, q in enumerate(ids)]
         itemsB = [item("t", stack, False)]
         if not groups:  # the groups share one shape: tune the first, reuse its tiles
-            tuneB, _ = A.Plan.autotune(itemsB, L.ADZ_F32, math=L.ADZ_MATH_F32, post=A.ADZ_POST_STORE, reps=2)
+            tuneB = fixed if fixed else A.Plan.autotune(itemsB, L.ADZ_F32, math=L.ADZ_MATH_F32,
+                                                        post=A.ADZ_POST_STORE, reps=2)[0]
         plan = A.Plan(itemsB, L.ADZ_F32, math=L.ADZ_MATH_F32, post=A.ADZ_POST_STORE, tune=tuneB)
         bcs = [dict(p=src[q], m=dst[q], lin=post[k]["lin"], min=post[k]["min"], max=post[k]["max"], mass=1)
                for k, q in enumerate(ids)]
@@ -199,7 +204,8 @@ def main():
         y = torch.empty_like(x)
         y.copy_(x)
         torch.cuda.synchronize()
-        print(json.dumps({"profile_kernel": "k_plan (tracer group)", "copy_bytes": 2 * 4 * 2 ** 28}), flush=True)
+        print(json.dumps({"profile_kernel": "k_plan (tracer group)", "tiles": tuneB, "copy_bytes": 2 * 4 * 2 ** 28}),
+              flush=True)
         return
 
     # ------------------------------------------------------------------ timed step (hipGraph)
