@@ -51,55 +51,75 @@ struct PlanArgs {
   int stage;
 };
 
-struct InlineFields {
+struct InlineFields {  // kernel-argument table (adz_tricub_hlt)
   const void* src[kInlineFields];
   void* dst[kInlineFields];
   void* lin[kInlineFields];
   void* mn[kInlineFields];
   void* mx[kInlineFields];
+  __device__ const void* s(int f) const { return src[f]; }
+  __device__ void* d(int f) const { return dst[f]; }
+  __device__ void* l(int f) const { return lin[f]; }
+  __device__ void* n(int f) const { return mn[f]; }
+  __device__ void* x(int f) const { return mx[f]; }
+};
+
+struct ItemFields {  // field pointer tables of a plan item (device arrays in global memory)
+  const void* const* src;
+  void* const* dst;
+  void* const* lin;
+  void* const* mn;
+  void* const* mx;
+  typedef const void* cvp;
+  typedef void* vp;
+  __device__ const void* s(int f) const { return ((const cvp ADZ_GLOBAL*)src)[f]; }
+  __device__ void* d(int f) const { return ((const vp ADZ_GLOBAL*)dst)[f]; }
+  __device__ void* l(int f) const { return ((const vp ADZ_GLOBAL*)lin)[f]; }
+  __device__ void* n(int f) const { return ((const vp ADZ_GLOBAL*)mn)[f]; }
+  __device__ void* x(int f) const { return ((const vp ADZ_GLOBAL*)mx)[f]; }
 };
 
 // Evaluate one target from a source addressed with row stride `ni` and plane stride `nij`
 // (global memory or the LDS box) and store the outputs at dst offset o.
-template <class TF, int MATH, int POST, class Fields, class SrcT>
-__device__ __forceinline__ void eval_store(const Fields& F, int f, const SrcT* f1p, long ni, long nij, int zlinear,
+template <class TF, int MATH, int POST, class Fields, class P>
+__device__ __forceinline__ void eval_store(const Fields& F, int f, P f1p, long ni, long nij, int zlinear,
                                            const void* cw, long o) {
   const long o1 = zlinear ? 0 : nij;
   const long o2 = o1 + nij, o3 = o2 + o1;
-  TF* d = (TF*)F.dst[f];
+  typename GPtr<TF>::type d = (typename GPtr<TF>::type)F.d(f);
   if (MATH == ADZ_MATH_REF) {
     const double* c = (const double*)cw;
-    double v = tricub_ref<SrcT>(f1p, c, ni, o1, o2, o3);
+    double v = tricub_ref<TF>(f1p, c, ni, o1, o2, o3);
     if (POST) {
       double l, mi, ma;
-      mono_extra<SrcT, double>(f1p, c, ni, o1, o2, &l, &mi, &ma);
+      mono_extra<TF, double>(f1p, c, ni, o1, o2, &l, &mi, &ma);
       // adz_interp_hlt.F90:158-160 narrows each to real before clipping lin
       TF ls = from_f64<TF>(l), mis = from_f64<TF>(mi), mas = from_f64<TF>(ma);
       TF vs = from_f64<TF>(v);
       if (POST & ADZ_POST_CLIP) vs = from_f64<TF>(fmax(to_f64(mis), fmin(to_f64(mas), to_f64(vs))));
       d[o] = vs;
       if (POST & ADZ_POST_STORE) {
-        ((TF*)F.lin[f])[o] = from_f64<TF>(fmax(to_f64(mis), fmin(to_f64(mas), to_f64(ls))));
-        ((TF*)F.mn[f])[o] = mis;
-        ((TF*)F.mx[f])[o] = mas;
+        ((typename GPtr<TF>::type)F.l(f))[o] = from_f64<TF>(fmax(to_f64(mis), fmin(to_f64(mas), to_f64(ls))));
+        ((typename GPtr<TF>::type)F.n(f))[o] = mis;
+        ((typename GPtr<TF>::type)F.x(f))[o] = mas;
       }
     } else {
       d[o] = from_f64<TF>(v);
     }
   } else {
     const float* c = (const float*)cw;
-    float v = tricub_f32<SrcT>(f1p, c, ni, o1, o2, o3);
+    float v = tricub_f32<TF>(f1p, c, ni, o1, o2, o3);
     if (POST) {
       float l, mi, ma;
-      mono_extra<SrcT, float>(f1p, c, ni, o1, o2, &l, &mi, &ma);
+      mono_extra<TF, float>(f1p, c, ni, o1, o2, &l, &mi, &ma);
       TF ls = from_f32<TF>(l), mis = from_f32<TF>(mi), mas = from_f32<TF>(ma);
       TF vs = from_f32<TF>(v);
       if (POST & ADZ_POST_CLIP) vs = from_f32<TF>(fmaxf(to_f32(mis), fminf(to_f32(mas), to_f32(vs))));
       d[o] = vs;
       if (POST & ADZ_POST_STORE) {
-        ((TF*)F.lin[f])[o] = from_f32<TF>(fmaxf(to_f32(mis), fminf(to_f32(mas), to_f32(ls))));
-        ((TF*)F.mn[f])[o] = mis;
-        ((TF*)F.mx[f])[o] = mas;
+        ((typename GPtr<TF>::type)F.l(f))[o] = from_f32<TF>(fmaxf(to_f32(mis), fminf(to_f32(mas), to_f32(ls))));
+        ((typename GPtr<TF>::type)F.n(f))[o] = mis;
+        ((typename GPtr<TF>::type)F.x(f))[o] = mas;
       }
     } else {
       d[o] = from_f32<TF>(v);
@@ -110,8 +130,8 @@ __device__ __forceinline__ void eval_store(const Fields& F, int f, const SrcT* f
 template <class TF, int MATH, int POST, class Fields>
 __device__ __forceinline__ void store_nan(const Fields& F, int f, long o) {
   TF nan = (MATH == ADZ_MATH_REF) ? from_f64<TF>(__builtin_nan("")) : from_f32<TF>(__builtin_nanf(""));
-  ((TF*)F.dst[f])[o] = nan;
-  if (POST & ADZ_POST_STORE) { ((TF*)F.lin[f])[o] = nan; ((TF*)F.mn[f])[o] = nan; ((TF*)F.mx[f])[o] = nan; }
+  ((typename GPtr<TF>::type)F.d(f))[o] = nan;
+  if (POST & ADZ_POST_STORE) { ((typename GPtr<TF>::type)F.l(f))[o] = nan; ((typename GPtr<TF>::type)F.n(f))[o] = nan; ((typename GPtr<TF>::type)F.x(f))[o] = nan; }
 }
 
 // block-wide minimum of 6 values over 256 lanes (4 waves of 64); `red` = 24 ints of LDS
@@ -185,7 +205,7 @@ __device__ __forceinline__ void run_tile(const DevItem& it, const Fields& F, lon
     constexpr int U = sizeof(TF) == 2 ? 2 : 4;          // bytes per lane per load
     const int units = nxb * (int)sizeof(TF) / U;         // loads per row
     auto issue = [&](int f, TF* buf) {
-      const char* src = (const char*)((const TF*)F.src[f] + gbase);
+      const char* src = (const char*)((const TF*)F.s(f) + gbase);
       char* dst = (char*)buf;
       for (int r = wv; r < nrows; r += kTJ) {
         const char* grow = src + (long)rowoff[r] * sizeof(TF);
@@ -212,32 +232,27 @@ __device__ __forceinline__ void run_tile(const DevItem& it, const Fields& F, lon
     __syncthreads();
     for (int f = f0; f < f1; f += 2) {
       if (f + 1 < f1) issue(f + 1, boxB);
-      if (ok) eval_store<TF, MATH, POST, Fields, TF>(F, f, boxA + boff, kPitch, nijb, x.zlinear, cw, o);
+      if (ok) eval_store<TF, MATH, POST, Fields>(F, f, boxA + boff, kPitch, nijb, x.zlinear, cw, o);
       else if (valid) store_nan<TF, MATH, POST>(F, f, o);
       asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
       __syncthreads();
       if (f + 1 >= f1) break;
       if (f + 2 < f1) issue(f + 2, boxA);
-      if (ok) eval_store<TF, MATH, POST, Fields, TF>(F, f + 1, boxB + boff, kPitch, nijb, x.zlinear, cw, o);
+      if (ok) eval_store<TF, MATH, POST, Fields>(F, f + 1, boxB + boff, kPitch, nijb, x.zlinear, cw, o);
       else if (valid) store_nan<TF, MATH, POST>(F, f + 1, o);
       asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
       __syncthreads();
     }
   } else {
     for (int f = f0; f < f1; f++) {
-      if (ok) eval_store<TF, MATH, POST, Fields, TF>(F, f, (const TF*)F.src[f] + x.off, gni, gnij, x.zlinear, cw, o);
+      if (ok)
+        eval_store<TF, MATH, POST, Fields>(F, f, (typename GPtr<TF>::ctype)F.s(f) + x.off, gni, gnij, x.zlinear, cw, o);
       else if (valid) store_nan<TF, MATH, POST>(F, f, o);
     }
   }
 }
 
-struct ItemFields {  // field pointer tables of a plan item (device arrays)
-  const void* const* src;
-  void* const* dst;
-  void* const* lin;
-  void* const* mn;
-  void* const* mx;
-};
+
 
 // Persistent kernel over the tile list of a plan.
 template <class TF, class TP, int MATH, int POST>
@@ -257,8 +272,11 @@ __global__ __launch_bounds__(kTile) void k_plan(PlanArgs A) {
   }
   for (long t = t0; t < hi; t += step) {
     int q = 0;
-    while (q + 1 < A.nitems && A.items[q + 1].tile0 <= t) q++;
-    const DevItem& it = A.items[q];
+    // the round trip through the global address space lets address-space inference turn
+    // the descriptor reads into global (not flat) loads
+    const DevItem* items = (const DevItem*)(const ADZ_GLOBAL DevItem*)A.items;
+    while (q + 1 < A.nitems && items[q + 1].tile0 <= t) q++;
+    const DevItem& it = items[q];
     long local = t - it.tile0;
     long ptile, grp;
     if (A.points_outer) { ptile = local / it.ngroups; grp = local % it.ngroups; }

@@ -124,13 +124,18 @@ __global__ __launch_bounds__(kBcBlock) void k_bc_pass(BcArgs A) {
   const double lambda = sc[SC_LAMBDA], prop = sc[SC_PROP];
   DD s0{0.0, 0.0}, s1{0.0, 0.0};
   const int r0 = (int)((long)A.nrows * blockIdx.y / A.nblk), r1 = (int)((long)A.nrows * (blockIdx.y + 1) / A.nblk);
-  const T* air1 = (const T*)A.airm1;
-  const T* air0 = (const T*)A.airm0;
+  const ADZ_GLOBAL T* air1 = (const ADZ_GLOBAL T*)A.airm1;
+  const ADZ_GLOBAL T* air0 = (const ADZ_GLOBAL T*)A.airm0;
+  const ADZ_GLOBAL double* maskg = (const ADZ_GLOBAL double*)A.mask;
+  ADZ_GLOBAL T* mg = (ADZ_GLOBAL T*)t.m;
   // Each lane gathers the inputs of kU elements (kU * 256 apart along the row) before it
   // accumulates them in order, so a wave keeps kU loads per array in flight.
   constexpr int kU = 4;
-  const T* lin = (const T*)t.lin;
-  const T* pk = (const T*)A.pkps;
+  const ADZ_GLOBAL T* lin = (const ADZ_GLOBAL T*)t.lin;
+  const ADZ_GLOBAL T* pk = (const ADZ_GLOBAL T*)A.pkps;
+  const ADZ_GLOBAL T* pg = (const ADZ_GLOBAL T*)t.p;
+  const ADZ_GLOBAL T* mng = (const ADZ_GLOBAL T*)t.mn;
+  const ADZ_GLOBAL T* mxg = (const ADZ_GLOBAL T*)t.mx;
   for (int r = r0; r < r1; r++) {
     const int j = A.j0 + r % A.ny, k = A.k0 + r / A.ny;
     const long q2r = (long)(A.i0 - 1) + (long)A.nxa * (j - 1);
@@ -143,21 +148,21 @@ __global__ __launch_bounds__(kBcBlock) void k_bc_pass(BcArgs A) {
         const int ii = ib + u * kBcBlock + (int)threadIdx.x;
         const bool in = ii < A.nx;
         const long q = qr + (in ? ii : 0), q2 = q2r + (in ? ii : 0);
-        mk[u] = in ? A.mask[q2] : 0.0;
-        mv[u] = ((T*)t.m)[q];
+        mk[u] = in ? maskg[q2] : 0.0;
+        mv[u] = mg[q];
         av[u] = air0[q];
-        if (PASS == P_PM) { bv[u] = air1[q]; lv[u] = ((const T*)t.p)[q]; }
+        if (PASS == P_PM) { bv[u] = air1[q]; lv[u] = pg[q]; }
         if (PASS == P_W || (PASS == P_CORR && !cyc)) {
           lv[u] = lin[q];
           pkv[u] = (t.weight == 3) ? (double)pk[q] : 0.0;
         }
-        if (PASS == P_CORR && !cyc && A.min_max) { nv[u] = ((const T*)t.mn)[q]; xv[u] = ((const T*)t.mx)[q]; }
+        if (PASS == P_CORR && !cyc && A.min_max) { nv[u] = mng[q]; xv[u] = mxg[q]; }
       }
 #pragma unroll
       for (int u = 0; u < kU; u++) {
         const int ii = ib + u * kBcBlock + (int)threadIdx.x;
         if (ii >= A.nx) break;
-        T* mp = (T*)t.m + qr + ii;
+        ADZ_GLOBAL T* mp = mg + qr + ii;
         T m = mv[u];
         if (PASS == P_PM) {
           dd_add(&s0, mass_term<T>(lv[u], bv[u], mk[u]));

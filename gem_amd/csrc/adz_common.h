@@ -24,6 +24,11 @@ int fail(int code, const std::string& msg);
 // ------------------------------------------------------------------ storage types
 typedef __hip_bfloat16 bf16;
 
+// Pointers read from device tables are generic; casting them into the global address
+// space lets the compiler emit global_load/global_store instead of flat accesses (which
+// also count against the LDS counter and force coarser waits).
+#define ADZ_GLOBAL __attribute__((address_space(1)))
+
 template <class T> __device__ __forceinline__ float to_f32(T v);
 template <> __device__ __forceinline__ float to_f32<float>(float v) { return v; }
 template <> __device__ __forceinline__ float to_f32<double>(double v) { return (float)v; }
@@ -33,6 +38,10 @@ template <> __device__ __forceinline__ float to_f32<bf16>(bf16 v) {
 }
 template <class T> __device__ __forceinline__ double to_f64(T v) { return (double)to_f32<T>(v); }
 template <> __device__ __forceinline__ double to_f64<double>(double v) { return v; }
+// Global-address-space pointer types of a storage type.  bf16 is a class whose members
+// expect generic objects, so it keeps generic pointers.
+template <class TF> struct GPtr { using type = ADZ_GLOBAL TF*; using ctype = const ADZ_GLOBAL TF*; };
+template <> struct GPtr<bf16> { using type = bf16*; using ctype = const bf16*; };
 
 // round-to-nearest-even FP32 -> BF16 (NaN kept quiet)
 __device__ __forceinline__ bf16 f32_to_bf16(float f) {
@@ -212,7 +221,7 @@ namespace adz {
 
 template <class TP>
 __device__ __forceinline__ void load_pos(const void* pxyz, long p, double* x, double* y, double* z) {
-  const TP* q = (const TP*)pxyz + 3 * p;
+  const TP* q = (const TP*)(const ADZ_GLOBAL TP*)pxyz + 3 * p;  // positions live in global memory
   *x = (double)q[0]; *y = (double)q[1]; *z = (double)q[2];
 }
 
