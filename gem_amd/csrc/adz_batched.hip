@@ -272,9 +272,7 @@ __global__ __launch_bounds__(kTile) void k_plan(PlanArgs A) {
   }
   for (long t = t0; t < hi; t += step) {
     int q = 0;
-    // the round trip through the global address space lets address-space inference turn
-    // the descriptor reads into global (not flat) loads
-    const DevItem* items = (const DevItem*)(const ADZ_GLOBAL DevItem*)A.items;
+    const DevItem* items = A.items;  // a kernel argument: the compiler reads it with global loads
     while (q + 1 < A.nitems && items[q + 1].tile0 <= t) q++;
     const DevItem& it = items[q];
     long local = t - it.tile0;

@@ -119,8 +119,8 @@ __device__ __forceinline__ void weights_ref(const Index& x, const ZtabDev& lv, d
   if (x.zlinear) {
     c[8] = 0.0; c[9] = 1.0 - dz; c[10] = dz; c[11] = 0.0;
   } else {
-    const double* base = lv.ocz + 4 * x.iz0;
-    const double* pos = lv.z + x.iz0;
+    const ADZ_GLOBAL double* base = (const ADZ_GLOBAL double*)lv.ocz + 4 * x.iz0;
+    const ADZ_GLOBAL double* pos = (const ADZ_GLOBAL double*)lv.z + x.iz0;
     double za = fma(dz, pos[1], (1.0 - dz) * pos[0]);
     double zza = za - pos[-1], zzb = za - pos[0], zzc = za - pos[1], zzd = za - pos[2];
     double zzab = zza * zzb, zzcd = zzc * zzd;
@@ -151,8 +151,8 @@ __device__ __forceinline__ void weights_f32(const Index& x, const ZtabDev& lv, f
   if (x.zlinear) {
     c[8] = 0.0f; c[9] = 1.0f - dz; c[10] = dz; c[11] = 0.0f;
   } else {
-    const float* h = lv.hz + 4 * x.iz0;
-    const float* o = lv.oczf + 4 * x.iz0;
+    const ADZ_GLOBAL float* h = (const ADZ_GLOBAL float*)lv.hz + 4 * x.iz0;
+    const ADZ_GLOBAL float* o = (const ADZ_GLOBAL float*)lv.oczf + 4 * x.iz0;
     float zzb = dz * h[1];
     float zza = zzb + h[0];
     float zzc = zzb - h[1];
@@ -221,7 +221,7 @@ namespace adz {
 
 template <class TP>
 __device__ __forceinline__ void load_pos(const void* pxyz, long p, double* x, double* y, double* z) {
-  const TP* q = (const TP*)(const ADZ_GLOBAL TP*)pxyz + 3 * p;  // positions live in global memory
+  const ADZ_GLOBAL TP* q = (const ADZ_GLOBAL TP*)pxyz + 3 * p;  // positions live in global memory
   *x = (double)q[0]; *y = (double)q[1]; *z = (double)q[2];
 }
 
