@@ -252,6 +252,7 @@ def main():
     ng = len(ids0) if ids0 else 0
     alg_plan0 = ng * 4 * (n_mod + Nt) + ng * 3 * 4 * Nt + 24 * Nt
     achieved = alg_plan0 / (t_plan0 * 1e-3) / 1e9
+    traffic = measured_traffic(args.config, tuneB)
     # interpolation-only throughput over all 64 fields
     alg_interp = (3 * 4 * n_ext + (nf - 3) * 4 * n_mod + 4 * pts_total + ntr * 12 * Nt +
                   24 * (npts(boxes["u"][0]) + npts(boxes["v"][0]) + Nt))
@@ -340,7 +341,7 @@ def main():
         "speedup_fp32_vs_fp64_plain": round(f32_gpts / plain_gpts, 3),
         "error_vs_fp64": {"worst_max_rel": worst_max, "worst_rms_rel": worst_rms, "per_field_max_rms": errs},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK, 4), "traffic": traffic,
                      "kernel": f"k_plan<float,double,F32,post> tracer group ({ng} fields), "
                                f"{t_plan0:.3f} ms/launch, {alg_plan0 / 1e9:.2f} GB algorithmic"},
         "cpu_baseline": cpu,
@@ -493,6 +494,21 @@ def run_c4(args, np, torch, L, A, S, dev):
     print(json.dumps(out), flush=True)
     plan.free()
     default_plan.free()
+
+
+def measured_traffic(config, tiles):
+    """HBM bytes per launch of the tracer-group kernel from the committed rocprofv3 PMC
+    summary (FETCH_SIZE and WRITE_SIZE, each scaled by the same counter's reading for a
+    copy of known size), when it was measured on this workload with these tiles."""
+    path = os.path.join(ROOT, "profiles", f"k_plan_{config}_pmc.json")
+    try:
+        with open(path) as f:
+            m = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if m.get("tiles") != tiles:
+        return None
+    return m.get("hbm_bytes_per_launch")
 
 
 def cpu_baseline(np, torch, cfg, g, src, pt, tbox, mod_shape, nf, seconds):
