@@ -20,6 +20,9 @@
 
 namespace adz {
 
+#ifndef ADZ_BC_UNROLL
+#define ADZ_BC_UNROLL 4  // elements gathered per lane before accumulating
+#endif
 constexpr int kBcBlock = 256;
 constexpr int kSc = 16;  // per-tracer scalars
 enum { SC_MP = 0, SC_MM, SC_MTP, SC_D, SC_W, SC_LAMBDA, SC_CYCLE, SC_MBCR, SC_PROP, SC_MBC };
@@ -130,7 +133,7 @@ __global__ __launch_bounds__(kBcBlock) void k_bc_pass(BcArgs A) {
   ADZ_GLOBAL T* mg = (ADZ_GLOBAL T*)t.m;
   // Each lane gathers the inputs of kU elements (kU * 256 apart along the row) before it
   // accumulates them in order, so a wave keeps kU loads per array in flight.
-  constexpr int kU = 4;
+  constexpr int kU = ADZ_BC_UNROLL;
   const ADZ_GLOBAL T* lin = (const ADZ_GLOBAL T*)t.lin;
   const ADZ_GLOBAL T* pk = (const ADZ_GLOBAL T*)A.pkps;
   const ADZ_GLOBAL T* pg = (const ADZ_GLOBAL T*)t.p;
