@@ -47,9 +47,8 @@ def parse():
     p.add_argument("--cpu-seconds", type=float, default=12.0)
     p.add_argument("--profile-kernel", action="store_true",
                    help="only build the workload and run the dominant interpolation kernel 5 times (rocprofv3)")
-    p.add_argument("--tiles", default='{"fields_per_pass": 20, "points_outer": 1, "blocks_per_cu": 8, '
-                                      '"xcd_chunks": 1, "stage": 2}',
-                   help="tracer-group tiles used by --profile-kernel (JSON; the timed run autotunes)")
+    p.add_argument("--retune", action="store_true",
+                   help="run adz_plan_autotune instead of using the tiles it found before (gem_amd/tuned_tiles.json)")
     return p.parse_args()
 
 
@@ -144,8 +143,8 @@ def main():
               item("t", [dict(src=src[2], dst=dst[2])], True),
               item("t", [dict(src=src[3], dst=dst[3])], False)]
     t_tune = time.time()
-    fixed = json.loads(args.tiles) if args.profile_kernel else None  # no tuning runs inside a profile
-    tuneA = fixed if fixed else A.Plan.autotune(itemsA, L.ADZ_F32, math=L.ADZ_MATH_F32, post=0, reps=2)[0]
+    cached = None if args.retune else tuned_tiles(args.config)  # the autotuner's earlier result
+    tuneA = cached["uvtq"] if cached else A.Plan.autotune(itemsA, L.ADZ_F32, math=L.ADZ_MATH_F32, post=0, reps=2)[0]
     planA = A.Plan(itemsA, L.ADZ_F32, math=L.ADZ_MATH_F32, post=0, tune=tuneA)
     # plans B: tracer groups with lin/min/max for the fixer, and the fixer itself
     groups = []
@@ -158,15 +157,16 @@ def main():
                  for k, q in enumerate(ids)]
         itemsB = [item("t", stack, False)]
         if not groups:  # the groups share one shape: tune the first, reuse its tiles
-            tuneB = fixed if fixed else A.Plan.autotune(itemsB, L.ADZ_F32, math=L.ADZ_MATH_F32,
-                                                        post=A.ADZ_POST_STORE, reps=2)[0]
+            tuneB = cached["tracer_groups"] if cached else A.Plan.autotune(
+                itemsB, L.ADZ_F32, math=L.ADZ_MATH_F32, post=A.ADZ_POST_STORE, reps=2)[0]
         plan = A.Plan(itemsB, L.ADZ_F32, math=L.ADZ_MATH_F32, post=A.ADZ_POST_STORE, tune=tuneB)
         bcs = [dict(p=src[q], m=dst[q], lin=post[k]["lin"], min=post[k]["min"], max=post[k]["max"], mass=1)
                for k, q in enumerate(ids)]
         bc = A.BermejoConde(bcs, layout5, scope, mod_off, min_max=True)
         groups.append((ids, plan, bc))
     tuneB = tuneB if groups else None
-    log(f"autotuned tiles ({time.time() - t_tune:.1f}s): u/v/T/q plan {tuneA}, tracer groups {tuneB}")
+    log(f"tiles ({'tuned now' if not cached else 'from gem_amd/tuned_tiles.json'}, {time.time() - t_tune:.1f}s): "
+        f"u/v/T/q plan {tuneA}, tracer groups {tuneB}")
     pts_total = npts(boxes["u"][0]) + npts(boxes["v"][0]) + (nf - 2) * npts(tbox)
     log(f"setup {time.time() - t_setup:.1f}s; {pts_total / 1e9:.3f} G interpolated point-fields per step")
 
@@ -392,8 +392,13 @@ def run_c4(args, np, torch, L, A, S, dev):
     log(f"c4: {len(state)} subdomains, {pts / 1e6:.1f} M target points, {pf / 1e9:.2f} G point-fields per step, "
         f"setup {time.time() - t_setup:.1f}s")
     t0 = time.time()
-    tune, tune_ms = A.Plan.autotune(items, L.ADZ_F32, math=L.ADZ_MATH_F32, reps=2)
-    log(f"autotune: {tune} -> {tune_ms:.2f} ms per launch ({time.time() - t0:.1f}s)")
+    cached = None if args.retune else tuned_tiles("c4")
+    if cached:
+        tune = cached["plan"]
+        log(f"tiles from gem_amd/tuned_tiles.json: {tune}")
+    else:
+        tune, tune_ms = A.Plan.autotune(items, L.ADZ_F32, math=L.ADZ_MATH_F32, reps=2)
+        log(f"autotune: {tune} -> {tune_ms:.2f} ms per launch ({time.time() - t0:.1f}s)")
     plan = A.Plan(items, L.ADZ_F32, math=L.ADZ_MATH_F32, tune=tune)
     default_plan = A.Plan(items, L.ADZ_F32, math=L.ADZ_MATH_F32)
 
@@ -494,6 +499,16 @@ def run_c4(args, np, torch, L, A, S, dev):
     print(json.dumps(out), flush=True)
     plan.free()
     default_plan.free()
+
+
+def tuned_tiles(config):
+    """Tile parameters adz_plan_autotune found for this workload (committed beside the
+    package), or None."""
+    try:
+        with open(os.path.join(ROOT, "gem_amd", "tuned_tiles.json")) as f:
+            return json.load(f).get(config)
+    except (OSError, ValueError):
+        return None
 
 
 def measured_traffic(config, tiles):
