@@ -159,7 +159,7 @@ class AdzItem(C.Structure):
 
 class AdzTune(C.Structure):
     _fields_ = [("fields_per_pass", C.c_int), ("points_outer", C.c_int), ("blocks_per_cu", C.c_int),
-                ("xcd_chunks", C.c_int), ("stage", C.c_int)]
+                ("xcd_chunks", C.c_int), ("stage", C.c_int), ("level_inner", C.c_int)]
 
 
 def _ptr_at(t, off):

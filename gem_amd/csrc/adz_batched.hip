@@ -49,6 +49,7 @@ struct PlanArgs {
   int points_outer;
   int xcd_chunks;
   int stage;
+  int level_inner;
 };
 
 struct InlineFields {  // kernel-argument table (adz_tricub_hlt)
@@ -155,12 +156,19 @@ __device__ __forceinline__ void block_min6(int* v, int* red) {
 // per-lane global loads.  Both paths run the same arithmetic in the same order.
 template <class TF, class TP, int MATH, int POST, class Fields>
 __device__ __forceinline__ void run_tile(const DevItem& it, const Fields& F, long ptile, int f0, int f1, TF* boxA,
-                                         TF* boxB, int* red, int* rowoff, int stage) {
+                                         TF* boxB, int* red, int* rowoff, int stage, int level_inner) {
   const int lane = threadIdx.x & (kTI - 1), wv = threadIdx.x / kTI;
   const long nib = (it.nx + kTI - 1) / kTI, njb = (it.ny + kTJ - 1) / kTJ;
-  const int ib = (int)(ptile % nib);
-  const int jb = (int)((ptile / nib) % njb);
-  const int kk = (int)(ptile / (nib * njb));
+  int ib, jb, kk;
+  if (level_inner) {  // consecutive tiles: the levels of one column (vertical reuse in L2)
+    kk = (int)(ptile % it.nz);
+    ib = (int)((ptile / it.nz) % nib);
+    jb = (int)(ptile / (it.nz * nib));
+  } else {            // consecutive tiles: along i, then j, then k
+    ib = (int)(ptile % nib);
+    jb = (int)((ptile / nib) % njb);
+    kk = (int)(ptile / (nib * njb));
+  }
   const int ii = ib * kTI + lane, jj = jb * kTJ + wv;
   const bool valid = ii < it.nx && jj < it.ny;
   const int i = it.i0 + ii, j = it.j0 + jj, k = it.k0 + kk;
@@ -281,7 +289,8 @@ __global__ __launch_bounds__(kTile) void k_plan(PlanArgs A) {
     else { grp = local / it.ptiles; ptile = local % it.ptiles; }
     int f0 = (int)grp * A.g, f1 = min(it.nf, f0 + A.g);
     ItemFields F{it.src, it.dst, it.lin, it.mn, it.mx};
-    run_tile<TF, TP, MATH, POST, ItemFields>(it, F, ptile, f0, f1, boxA, boxB, red, rowoff, A.stage);
+    run_tile<TF, TP, MATH, POST, ItemFields>(it, F, ptile, f0, f1, boxA, boxB, red, rowoff, A.stage,
+                                             A.level_inner);
   }
 }
 
@@ -292,7 +301,7 @@ __global__ __launch_bounds__(kTile) void k_hlt(DevItem it, InlineFields F) {
   __shared__ TF boxB[kMaxRows * kPitch];
   __shared__ int red[24];
   __shared__ int rowoff[kMaxRows];
-  run_tile<TF, TP, MATH, POST, InlineFields>(it, F, blockIdx.x, 0, it.nf, boxA, boxB, red, rowoff, 2);
+  run_tile<TF, TP, MATH, POST, InlineFields>(it, F, blockIdx.x, 0, it.nf, boxA, boxB, red, rowoff, 2, 0);
 }
 
 }  // namespace adz
@@ -422,13 +431,14 @@ int adz_plan_create(const adz_item* items, int nitems, int fdtype, int math, int
   for (int q = 0; q < nitems; q++)
     if (items[q].pdtype != pdt) return fail(ADZ_EINVAL, "adz_plan_create: items must share the position dtype");
   if (!pick<true>(fdtype, pdt, math, post)) return fail(ADZ_EINVAL, "adz_plan_create: unsupported dtype/math");
-  adz_tune T{8, 0, 4, 1, 2};
+  adz_tune T{8, 0, 4, 1, 2, 0};
   if (tune) {
     if (tune->fields_per_pass > 0) T.fields_per_pass = std::min(64, tune->fields_per_pass);
     T.points_outer = tune->points_outer;
     if (tune->blocks_per_cu > 0) T.blocks_per_cu = tune->blocks_per_cu;
     T.xcd_chunks = tune->xcd_chunks;
     if (tune->stage > 0) T.stage = tune->stage;
+    T.level_inner = tune->level_inner;
   }
   std::vector<DevItem> dv(nitems);
   long nptr_total = 0;
@@ -479,7 +489,8 @@ int adz_plan_create(const adz_item* items, int nitems, int fdtype, int math, int
   }
   adz_plan* p = new adz_plan;
   p->fdtype = fdtype; p->pdtype = pdt; p->math = math; p->post = post; p->tune = T; p->dmem = mem;
-  p->args = PlanArgs{(const DevItem*)mem, nitems, tiles, T.fields_per_pass, T.points_outer, T.xcd_chunks, T.stage};
+  p->args = PlanArgs{(const DevItem*)mem, nitems, tiles, T.fields_per_pass, T.points_outer, T.xcd_chunks, T.stage,
+                     T.level_inner};
   long grid = (long)T.blocks_per_cu * device_cus();
   if (grid > tiles) grid = tiles;
   if (T.xcd_chunks) grid = std::max(8L, (grid / 8) * 8);
@@ -531,22 +542,27 @@ int adz_plan_autotune(const adz_item* items, int nitems, int fdtype, int math, i
     int rf = adz_plan_free(p);
     return rc != ADZ_OK ? rc : rf;
   };
-  adz_tune cur{8, 0, 4, 1, 2};
+  adz_tune cur{8, 0, 4, 1, 2, 0};
   cur.fields_per_pass = std::min(cur.fields_per_pass, maxf);
   float cur_ms = 0.f;
   int rc = time_one(cur, &cur_ms);
   const int stages[] = {1, 2}, fpps[] = {2, 4, 8, 16, 32, 64}, bpcs[] = {2, 4, 8}, flags[] = {0, 1};
-  for (int coord = 0; coord < 5 && rc == ADZ_OK; coord++) {
-    const int* vals = coord == 0 ? stages : coord == 1 ? fpps : coord == 2 ? bpcs : flags;
-    int nv = coord == 0 ? 2 : coord == 1 ? 6 : coord == 2 ? 3 : 2;
+  // coordinates: 0 staging, 1 tile order, 2 fields per pass, 3 blocks per CU, 4 loop order,
+  // 5 XCD chunking; the tile order is searched again after fields per pass (they interact
+  // through the L2 working set)
+  const int coords[] = {0, 1, 2, 1, 3, 4, 5};
+  for (int ci = 0; ci < 7 && rc == ADZ_OK; ci++) {
+    const int coord = coords[ci];
+    const int* vals = coord == 0 ? stages : coord == 2 ? fpps : coord == 3 ? bpcs : flags;
+    int nv = coord == 0 ? 2 : coord == 2 ? 6 : coord == 3 ? 3 : 2;
     for (int v = 0; v < nv && rc == ADZ_OK; v++) {
       adz_tune t = cur;
-      int* slot = coord == 0 ? &t.stage : coord == 1 ? &t.fields_per_pass : coord == 2 ? &t.blocks_per_cu
-                : coord == 3 ? &t.points_outer : &t.xcd_chunks;
+      int* slot = coord == 0 ? &t.stage : coord == 1 ? &t.level_inner : coord == 2 ? &t.fields_per_pass
+                : coord == 3 ? &t.blocks_per_cu : coord == 4 ? &t.points_outer : &t.xcd_chunks;
       if (*slot == vals[v]) continue;
-      if (coord == 1 && vals[v] > maxf && vals[v - 1] >= maxf) continue;  // one candidate covers all fields
-      *slot = coord == 1 ? std::min(vals[v], maxf) : vals[v];
-      if (coord == 1 && t.fields_per_pass == cur.fields_per_pass) continue;
+      if (coord == 2 && vals[v] > maxf && vals[v - 1] >= maxf) continue;  // one candidate covers all fields
+      *slot = coord == 2 ? std::min(vals[v], maxf) : vals[v];
+      if (coord == 2 && t.fields_per_pass == cur.fields_per_pass) continue;
       float ms = 0.f;
       rc = time_one(t, &ms);
       if (rc == ADZ_OK && ms < cur_ms) { cur = t; cur_ms = ms; }

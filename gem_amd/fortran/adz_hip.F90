@@ -45,6 +45,7 @@ module adz_hip
 
    type, bind(C) :: adz_tune
       integer(c_int) :: fields_per_pass = 0, points_outer = 0, blocks_per_cu = 0, xcd_chunks = 1, stage = 0
+      integer(c_int) :: level_inner = 0
    end type adz_tune
 
    type, bind(C) :: adz_bc_tracer

@@ -202,7 +202,7 @@ typedef struct adz_item {
 } adz_item;
 
 /* Tile parameters of the persistent kernel; zero fields take the built-in default
-   {8, 0, 4, 1, 2}; adz_plan_autotune searches them on the actual plan. */
+   {8, 0, 4, 1, 2, 0}; adz_plan_autotune searches them on the actual plan. */
 typedef struct adz_tune {
   int fields_per_pass; /* fields interpolated per pass over the points (1..64)          */
   int points_outer;    /* 1: a block runs every field group of a point tile; 0: a pass
@@ -210,6 +210,8 @@ typedef struct adz_tune {
   int blocks_per_cu;   /* persistent grid = blocks_per_cu * CUs                         */
   int xcd_chunks;      /* 1: each XCD takes a contiguous range of tiles                 */
   int stage;           /* 1: stage each tile's source box in LDS; 2: per-lane loads     */
+  int level_inner;     /* 1: consecutive tiles walk the levels of one column (a source
+                          plane feeds 4 levels: vertical reuse in L2); 0: i, j, then k   */
 } adz_tune;
 
 typedef struct adz_plan adz_plan;
@@ -222,7 +224,7 @@ int adz_plan_execute(const adz_plan* p, adz_stream stream);
 int adz_plan_tune(const adz_plan* p, adz_tune* out); /* the parameters in use */
 int adz_plan_free(adz_plan* p);
 /* Project autotuner for the persistent kernel's tile parameters: a coordinate search
- * (staging, fields per pass, blocks per CU, loop order, XCD chunking), each candidate
+ * (staging, tile order, fields per pass, blocks per CU, loop order, XCD chunking), each candidate
  * timed over `reps` executions of a plan built from `items`; the items' destinations are
  * written.  Returns the fastest tune and its time per execution (ms).  No counterpart in
  * the reference (its slices are fixed at ni*nj points, adz_interp_hlt.F90:102). */

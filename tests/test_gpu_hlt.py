@@ -156,7 +156,11 @@ def test_hlt_f32_math_error_vs_fp64_oracle(T):
 @pytest.mark.parametrize("tune", [None, dict(fields_per_pass=1, points_outer=0, blocks_per_cu=2, xcd_chunks=0),
                                   dict(fields_per_pass=3, points_outer=1, blocks_per_cu=4, xcd_chunks=1),
                                   dict(fields_per_pass=5, points_outer=0, blocks_per_cu=4, xcd_chunks=1, stage=1),
-                                  dict(fields_per_pass=2, points_outer=1, blocks_per_cu=8, xcd_chunks=0, stage=1)])
+                                  dict(fields_per_pass=2, points_outer=1, blocks_per_cu=8, xcd_chunks=0, stage=1),
+                                  dict(fields_per_pass=4, points_outer=1, blocks_per_cu=4, xcd_chunks=1, stage=2,
+                                       level_inner=1),
+                                  dict(fields_per_pass=3, points_outer=0, blocks_per_cu=4, xcd_chunks=1, stage=1,
+                                       level_inner=1)])
 def test_plan_matches_single_calls(T, tune):
     torch, L, A = T
     cfgs = [S.Config(f"p{s}", 40 + 16 * s, 36 + 8 * s, 16, 4, 300.0, jets=True, cmax_xy=2.0, seed=60 + s)
