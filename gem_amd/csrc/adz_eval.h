@@ -1,5 +1,7 @@
 // Per-target field evaluators shared by the interpolation kernels (gfx950).
 #pragma once
+#include <type_traits>
+
 #include "adz_common.h"
 
 namespace adz {
@@ -48,23 +50,39 @@ __device__ __forceinline__ float tricub_f32(P f1, const float* c, long ni, long 
 // min / max / linear of the monotone interpolator (turbo_tricublin.c:883-989): min/max
 // over x columns 1..2, y rows 0..3, z planes 1..2 with the MINPD/MAXPD operand rule; the
 // linear result from rows 1..2 and planes 1..2.
+// The FP32 form (TC = float) takes min / max with v_min3/v_max3 (IEEE minNum/maxNum):
+// the same values except where the operand rule matters (NaN, or +0 against -0).
 template <class TF, class TC, class P>
 __device__ __forceinline__ void mono_extra(P f1, const TC* c, long ni, long o1, long o2, TC* lin, TC* mi, TC* ma) {
-  TC vmi[4], vma[4];
+  if constexpr (std::is_same<TC, float>::value) {
+    float lo = to_f32<TF>(f1[1 + o1]), hi = lo;
 #pragma unroll
-  for (int r = 0; r < 4; r++) {
-    P s = f1 + r * ni;
-#pragma unroll
-    for (int i = 1; i < 3; i++) {
-      TC a = (TC)to_f64<TF>(s[i + o1]), b = (TC)to_f64<TF>(s[i + o2]);
-      if (r == 0) { vmi[i] = a; vma[i] = a; }
-      else { vmi[i] = (vmi[i] < a) ? vmi[i] : a; vma[i] = (vma[i] > a) ? vma[i] : a; }
-      vmi[i] = (vmi[i] < b) ? vmi[i] : b;
-      vma[i] = (vma[i] > b) ? vma[i] : b;
+    for (int r = 0; r < 4; r++) {
+      P s = f1 + r * ni;
+      float a1 = to_f32<TF>(s[1 + o1]), a2 = to_f32<TF>(s[2 + o1]);
+      float b1 = to_f32<TF>(s[1 + o2]), b2 = to_f32<TF>(s[2 + o2]);
+      lo = fminf(fminf(lo, a1), fminf(a2, fminf(b1, b2)));
+      hi = fmaxf(fmaxf(hi, a1), fmaxf(a2, fmaxf(b1, b2)));
     }
+    *mi = lo;
+    *ma = hi;
+  } else {
+    TC vmi[4], vma[4];
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      P s = f1 + r * ni;
+#pragma unroll
+      for (int i = 1; i < 3; i++) {
+        TC a = (TC)to_f64<TF>(s[i + o1]), b = (TC)to_f64<TF>(s[i + o2]);
+        if (r == 0) { vmi[i] = a; vma[i] = a; }
+        else { vmi[i] = (vmi[i] < a) ? vmi[i] : a; vma[i] = (vma[i] > a) ? vma[i] : a; }
+        vmi[i] = (vmi[i] < b) ? vmi[i] : b;
+        vma[i] = (vma[i] > b) ? vma[i] : b;
+      }
+    }
+    *mi = (vmi[1] < vmi[2]) ? vmi[1] : vmi[2];
+    *ma = (vma[1] > vma[2]) ? vma[1] : vma[2];
   }
-  *mi = (vmi[1] < vmi[2]) ? vmi[1] : vmi[2];
-  *ma = (vma[1] > vma[2]) ? vma[1] : vma[2];
   TC vyl[4];
 #pragma unroll
   for (int i = 0; i < 4; i++) {
