@@ -33,6 +33,9 @@ struct DevItem {
   int i0, j0, k0, nx, ny, nz;
   int dni, dnj;
   int nf;
+  long minoff;      // lowest source element a stencil may read, from element (1,1,1)
+  unsigned span;    // bytes of FP32 source from minoff that stencils may read (0: too large
+                    // for 32-bit buffer offsets; global loads are used)
   long npts, ptiles, ngroups, tile0;
   const void* const* src;
   void* const* dst;
@@ -80,20 +83,16 @@ struct ItemFields {  // field pointer tables of a plan item (device arrays in gl
   __device__ void* x(int f) const { return ((const vp ADZ_GLOBAL*)mx)[f]; }
 };
 
-// Evaluate one target from a source addressed with row stride `ni` and plane stride `nij`
-// (global memory or the LDS box) and store the outputs at dst offset o.
-template <class TF, int MATH, int POST, class Fields, class P>
-__device__ __forceinline__ void eval_store(const Fields& F, int f, P f1p, long ni, long nij, int zlinear,
-                                           const void* cw, long o) {
-  const long o1 = zlinear ? 0 : nij;
-  const long o2 = o1 + nij, o3 = o2 + o1;
+// Evaluate one target from its stencil accessor and store the outputs at dst offset o.
+template <class TF, int MATH, int POST, class Fields, class Acc>
+__device__ __forceinline__ void eval_store_a(const Fields& F, int f, Acc at, const void* cw, long o) {
   typename GPtr<TF>::type d = (typename GPtr<TF>::type)F.d(f);
   if (MATH == ADZ_MATH_REF) {
     const double* c = (const double*)cw;
-    double v = tricub_ref<TF>(f1p, c, ni, o1, o2, o3);
+    double v = tricub_ref_a<TF>(at, c);
     if (POST) {
       double l, mi, ma;
-      mono_extra<TF, double>(f1p, c, ni, o1, o2, &l, &mi, &ma);
+      mono_extra_a<TF, double>(at, c, &l, &mi, &ma);
       // adz_interp_hlt.F90:158-160 narrows each to real before clipping lin
       TF ls = from_f64<TF>(l), mis = from_f64<TF>(mi), mas = from_f64<TF>(ma);
       TF vs = from_f64<TF>(v);
@@ -109,10 +108,10 @@ __device__ __forceinline__ void eval_store(const Fields& F, int f, P f1p, long n
     }
   } else {
     const float* c = (const float*)cw;
-    float v = tricub_f32<TF>(f1p, c, ni, o1, o2, o3);
+    float v = tricub_f32_a<TF>(at, c);
     if (POST) {
       float l, mi, ma;
-      mono_extra<TF, float>(f1p, c, ni, o1, o2, &l, &mi, &ma);
+      mono_extra_a<TF, float>(at, c, &l, &mi, &ma);
       TF ls = from_f32<TF>(l), mis = from_f32<TF>(mi), mas = from_f32<TF>(ma);
       TF vs = from_f32<TF>(v);
       if (POST & ADZ_POST_CLIP) vs = from_f32<TF>(fmaxf(to_f32(mis), fminf(to_f32(mas), to_f32(vs))));
@@ -126,6 +125,34 @@ __device__ __forceinline__ void eval_store(const Fields& F, int f, P f1p, long n
       d[o] = from_f32<TF>(v);
     }
   }
+}
+
+// The same from a source addressed with row stride `ni` and plane stride `nij` (global
+// memory or the LDS box).
+template <class TF, int MATH, int POST, class Fields, class P>
+__device__ __forceinline__ void eval_store(const Fields& F, int f, P f1p, long ni, long nij, int zlinear,
+                                           const void* cw, long o) {
+  const long o1 = zlinear ? 0 : nij;
+  const long o2 = o1 + nij, o3 = o2 + o1;
+  eval_store_a<TF, MATH, POST>(F, f, PtrStencil<P>{f1p, ni, o1, o2, o3}, cw, o);
+}
+
+// FP32 sources through buffer loads: the 16 rows of a lane's stencil are read with
+// buffer_load_dwordx4 at 32-bit byte offsets from the lowest address a stencil may touch.
+// The offsets are computed once per target and reused for every field; per field only the
+// (scalar) buffer descriptor changes.
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+struct RegStencil {
+  const f32x4 (*v)[4];
+  __device__ __forceinline__ float operator()(int r, int z, int i) const { return v[r][z][i]; }
+};
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t field_rsrc(const void* base, unsigned bytes) {
+  // the field pointer is uniform across the wave: keep the descriptor in SGPRs
+  unsigned long long a = (unsigned long long)base;
+  unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)a), hi = __builtin_amdgcn_readfirstlane((unsigned)(a >> 32));
+  void* p = (void*)(((unsigned long long)hi << 32) | lo);
+  return __builtin_amdgcn_make_buffer_rsrc(p, (short)0, (int)bytes, 0x00020000);
 }
 
 template <class TF, int MATH, int POST, class Fields>
@@ -147,6 +174,48 @@ __device__ __forceinline__ void block_min6(int* v, int* red) {
 #pragma unroll
   for (int q = 0; q < 6; q++) v[q] = min(min(red[q * 4], red[q * 4 + 1]), min(red[q * 4 + 2], red[q * 4 + 3]));
   __syncthreads();
+}
+
+// Per-lane evaluation of fields [f0, f1) for one target: every lane reads its own 16
+// stencil rows (buffer loads for FP32 sources, global loads otherwise).
+template <class TF, int MATH, int POST, class Fields>
+__device__ __forceinline__ void per_lane(const DevItem& it, const Fields& F, int f0, int f1, const Index& x, bool ok,
+                                         bool valid, const double* cw, long o) {
+  const long gni = it.lv.ni, gnij = it.lv.nij;
+  if constexpr (std::is_same<TF, float>::value) {
+    if (it.span) {
+      int bo[4][4];
+      if (ok) {
+        const long o1 = x.zlinear ? 0 : gnij, o2 = o1 + gnij, o3 = o2 + o1;
+        const long e = x.off - it.minoff;
+#pragma unroll
+        for (int r = 0; r < 4; r++)
+#pragma unroll
+          for (int z = 0; z < 4; z++)
+            bo[r][z] = (int)((e + r * gni + (z == 0 ? 0 : (z == 1 ? o1 : (z == 2 ? o2 : o3)))) * 4);
+      }
+      for (int f = f0; f < f1; f++) {
+        if (ok) {
+          const __amdgpu_buffer_rsrc_t rs = field_rsrc((const float*)F.s(f) + it.minoff, it.span);
+          f32x4 v[4][4];
+#pragma unroll
+          for (int r = 0; r < 4; r++)
+#pragma unroll
+            for (int z = 0; z < 4; z++)
+              v[r][z] = __builtin_bit_cast(f32x4, __builtin_amdgcn_raw_buffer_load_b128(rs, bo[r][z], 0, 0));
+          eval_store_a<TF, MATH, POST>(F, f, RegStencil{v}, cw, o);
+        } else if (valid) {
+          store_nan<TF, MATH, POST>(F, f, o);
+        }
+      }
+      return;
+    }
+  }
+  for (int f = f0; f < f1; f++) {
+    if (ok)
+      eval_store<TF, MATH, POST, Fields>(F, f, (typename GPtr<TF>::ctype)F.s(f) + x.off, gni, gnij, x.zlinear, cw, o);
+    else if (valid) store_nan<TF, MATH, POST>(F, f, o);
+  }
 }
 
 // One tile: TI x TJ targets (i fastest) of one level of item `it`, fields [f0, f1).
@@ -186,6 +255,10 @@ __device__ __forceinline__ void run_tile(const DevItem& it, const Fields& F, lon
     if (MATH == ADZ_MATH_REF) weights_ref(x, it.lv, cw);
     else weights_f32(x, it.lv, (float*)cw);
   }
+  if (stage == 2) {  // per-lane loads: no box needed
+    per_lane<TF, MATH, POST>(it, F, f0, f1, x, ok, valid, cw, o);
+    return;
+  }
   // box = union of the tile's stencils (columns ix-1..ix+2, rows iy-1..iy+2, planes pz..)
   const int big = 1 << 30;
   int ext[6] = {ok ? x.ix - 1 : big, ok ? -(x.ix + 2) : big, ok ? x.iy - 1 : big, ok ? -(x.iy + 2) : big,
@@ -201,7 +274,7 @@ __device__ __forceinline__ void run_tile(const DevItem& it, const Fields& F, lon
   const long vol = (long)nxb * nyb * nzb;
   const long gni = it.lv.ni, gnij = it.lv.nij;
   const int nrows = nyb * nzb;
-  if (stage != 2 && nrows <= kMaxRows && nxb <= kPitch) {
+  if (nrows <= kMaxRows && nxb <= kPitch) {
     const long nijb = (long)kPitch * nyb;
     const long boff = ok ? (long)(x.ix - 1 - x0) + (long)(x.iy - 1 - y0) * kPitch + (long)(x.pz - z0) * nijb : 0;
     const long gbase = (long)(x0 - 1) + (long)(y0 - 1) * gni + (long)z0 * gnij;  // box corner from f(1,1,1)
@@ -252,23 +325,15 @@ __device__ __forceinline__ void run_tile(const DevItem& it, const Fields& F, lon
       __syncthreads();
     }
   } else {
-    for (int f = f0; f < f1; f++) {
-      if (ok)
-        eval_store<TF, MATH, POST, Fields>(F, f, (typename GPtr<TF>::ctype)F.s(f) + x.off, gni, gnij, x.zlinear, cw, o);
-      else if (valid) store_nan<TF, MATH, POST>(F, f, o);
-    }
+    per_lane<TF, MATH, POST>(it, F, f0, f1, x, ok, valid, cw, o);
   }
 }
 
 
 
-// Persistent kernel over the tile list of a plan.
-template <class TF, class TP, int MATH, int POST>
-__global__ __launch_bounds__(kTile) void k_plan(PlanArgs A) {
-  __shared__ TF boxA[kMaxRows * kPitch];
-  __shared__ TF boxB[kMaxRows * kPitch];
-  __shared__ int red[24];
-  __shared__ int rowoff[kMaxRows];
+// Persistent loop over the tile list of a plan.
+template <class TF, class TP, int MATH, int POST, int STAGE>
+__device__ __forceinline__ void plan_loop(const PlanArgs& A, TF* boxA, TF* boxB, int* red, int* rowoff) {
   long lo = 0, hi = A.ntiles, step = gridDim.x, t0 = blockIdx.x;
   if (A.xcd_chunks) {
     int nb = gridDim.x / 8;                 // blocks per XCD (grid is a multiple of 8)
@@ -289,19 +354,31 @@ __global__ __launch_bounds__(kTile) void k_plan(PlanArgs A) {
     else { grp = local / it.ptiles; ptile = local % it.ptiles; }
     int f0 = (int)grp * A.g, f1 = min(it.nf, f0 + A.g);
     ItemFields F{it.src, it.dst, it.lin, it.mn, it.mx};
-    run_tile<TF, TP, MATH, POST, ItemFields>(it, F, ptile, f0, f1, boxA, boxB, red, rowoff, A.stage,
-                                             A.level_inner);
+    run_tile<TF, TP, MATH, POST, ItemFields>(it, F, ptile, f0, f1, boxA, boxB, red, rowoff, STAGE, A.level_inner);
+  }
+}
+
+// Persistent kernel: STAGE 1 stages each tile's source box in LDS, STAGE 2 reads per lane
+// (separate instances, so the per-lane one carries neither the LDS buffers nor the
+// staging path's registers).
+template <class TF, class TP, int MATH, int POST, int STAGE>
+__global__ __launch_bounds__(kTile) void k_plan(PlanArgs A) {
+  if constexpr (STAGE == 1) {
+    __shared__ TF boxA[kMaxRows * kPitch];
+    __shared__ TF boxB[kMaxRows * kPitch];
+    __shared__ int red[24];
+    __shared__ int rowoff[kMaxRows];
+    plan_loop<TF, TP, MATH, POST, 1>(A, boxA, boxB, red, rowoff);
+  } else {
+    plan_loop<TF, TP, MATH, POST, 2>(A, nullptr, nullptr, nullptr, nullptr);
   }
 }
 
 // One item with inline field pointers (the adz_tricub_hlt entry point).
 template <class TF, class TP, int MATH, int POST>
 __global__ __launch_bounds__(kTile) void k_hlt(DevItem it, InlineFields F) {
-  __shared__ TF boxA[kMaxRows * kPitch];
-  __shared__ TF boxB[kMaxRows * kPitch];
-  __shared__ int red[24];
-  __shared__ int rowoff[kMaxRows];
-  run_tile<TF, TP, MATH, POST, InlineFields>(it, F, blockIdx.x, 0, it.nf, boxA, boxB, red, rowoff, 2, 0);
+  run_tile<TF, TP, MATH, POST, InlineFields>(it, F, blockIdx.x, 0, it.nf, (TF*)nullptr, (TF*)nullptr, nullptr,
+                                             nullptr, 2, 0);
 }
 
 }  // namespace adz
@@ -335,6 +412,13 @@ bool fill_item(DevItem* d, const adz_ztab* geom, const void* xyz, int i0, int in
   d->nx = in - i0 + 1; d->ny = jn - j0 + 1; d->nz = nk - k0 + 1;
   d->dni = dmaxx - dminx + 1; d->dnj = dmaxy - dminy + 1;
   d->nf = nf;
+  {
+    const ZtabDev& v = d->lv;
+    d->minoff = (long)(v.ilo - 1) + (long)(v.jlo - 1) * v.ni;
+    long maxoff = (long)(v.ihi - 1) + (long)(v.jhi - 1) * v.ni + (long)(v.nk - 1) * v.nij;
+    long bytes = (maxoff - d->minoff + 1) * 4;
+    d->span = (bytes > 0 && bytes < 0x7fffffffL) ? (unsigned)bytes : 0u;
+  }
   d->npts = (long)d->nx * d->ny * d->nz;
   d->ptiles = (long)((d->nx + kTI - 1) / kTI) * ((d->ny + kTJ - 1) / kTJ) * d->nz;
   return true;
@@ -342,12 +426,20 @@ bool fill_item(DevItem* d, const adz_ztab* geom, const void* xyz, int i0, int in
 
 // select the kernel instance for (fdtype, pdtype, math, post)
 template <class TF, class TP, int M>
-void* pick_plan(int post) {
+void* pick_plan(int post, int stage) {
+  if (stage == 1) {
+    switch (post) {
+      case 0: return (void*)&k_plan<TF, TP, M, 0, 1>;
+      case 1: return (void*)&k_plan<TF, TP, M, 1, 1>;
+      case 2: return (void*)&k_plan<TF, TP, M, 2, 1>;
+      default: return (void*)&k_plan<TF, TP, M, 3, 1>;
+    }
+  }
   switch (post) {
-    case 0: return (void*)&k_plan<TF, TP, M, 0>;
-    case 1: return (void*)&k_plan<TF, TP, M, 1>;
-    case 2: return (void*)&k_plan<TF, TP, M, 2>;
-    default: return (void*)&k_plan<TF, TP, M, 3>;
+    case 0: return (void*)&k_plan<TF, TP, M, 0, 2>;
+    case 1: return (void*)&k_plan<TF, TP, M, 1, 2>;
+    case 2: return (void*)&k_plan<TF, TP, M, 2, 2>;
+    default: return (void*)&k_plan<TF, TP, M, 3, 2>;
   }
 }
 template <class TF, class TP, int M>
@@ -361,8 +453,8 @@ void* pick_hlt(int post) {
 }
 
 template <bool PLAN>
-void* pick(int fdt, int pdt, int math, int post) {
-#define P_(TF, TP, M) return PLAN ? pick_plan<TF, TP, M>(post) : pick_hlt<TF, TP, M>(post)
+void* pick(int fdt, int pdt, int math, int post, int stage = 2) {
+#define P_(TF, TP, M) return PLAN ? pick_plan<TF, TP, M>(post, stage) : pick_hlt<TF, TP, M>(post)
   if (math == ADZ_MATH_REF) {
     if (fdt == ADZ_F64 && pdt == ADZ_F64) P_(double, double, ADZ_MATH_REF);
     if (fdt == ADZ_F64 && pdt == ADZ_F32) P_(double, float, ADZ_MATH_REF);
@@ -501,7 +593,7 @@ int adz_plan_create(const adz_item* items, int nitems, int fdtype, int math, int
 
 int adz_plan_execute(const adz_plan* p, adz_stream stream) {
   if (!p) return fail(ADZ_EINVAL, "adz_plan_execute: null plan");
-  void* kern = pick<true>(p->fdtype, p->pdtype, p->math, p->post);
+  void* kern = pick<true>(p->fdtype, p->pdtype, p->math, p->post, p->tune.stage);
   PlanArgs a = p->args;
   void* args[] = {&a};
   ADZ_HIP_CHECK(hipLaunchKernel(kern, dim3(p->grid), dim3(kTile), args, 0, (hipStream_t)stream));
