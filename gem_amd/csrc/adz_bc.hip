@@ -20,9 +20,6 @@
 
 namespace adz {
 
-#ifndef ADZ_BC_UNROLL
-#define ADZ_BC_UNROLL 4  // elements gathered per lane before accumulating
-#endif
 constexpr int kBcBlock = 256;
 constexpr int kSc = 16;  // per-tracer scalars
 enum { SC_MP = 0, SC_MM, SC_MTP, SC_D, SC_W, SC_LAMBDA, SC_CYCLE, SC_MBCR, SC_PROP, SC_MBC };
@@ -116,7 +113,7 @@ __device__ __forceinline__ double mass_term(T x, T air, double mask) {
 
 // Grid: x = tracer (fastest), y = block of scope rows, so the blocks of every tracer over
 // the same rows are dispatched together and share the air mass, mask and pkps through
-// the caches.  Each block owns whole rows (a fixed partition), lanes stride along i.
+// the caches.  Each block owns whole rows (a fixed partition).
 template <class T, int PASS>
 __global__ __launch_bounds__(kBcBlock) void k_bc_pass(BcArgs A) {
   const int n = blockIdx.x;
@@ -131,68 +128,94 @@ __global__ __launch_bounds__(kBcBlock) void k_bc_pass(BcArgs A) {
   const ADZ_GLOBAL T* air0 = (const ADZ_GLOBAL T*)A.airm0;
   const ADZ_GLOBAL double* maskg = (const ADZ_GLOBAL double*)A.mask;
   ADZ_GLOBAL T* mg = (ADZ_GLOBAL T*)t.m;
-  // Each lane gathers the inputs of kU elements (kU * 256 apart along the row) before it
-  // accumulates them in order, so a wave keeps kU loads per array in flight.
-  constexpr int kU = ADZ_BC_UNROLL;
+  // Each lane takes 4 consecutive elements of the row per batch (16-byte loads and stores
+  // for FP32), gathers all their inputs, then accumulates them in order.
   const ADZ_GLOBAL T* lin = (const ADZ_GLOBAL T*)t.lin;
   const ADZ_GLOBAL T* pk = (const ADZ_GLOBAL T*)A.pkps;
   const ADZ_GLOBAL T* pg = (const ADZ_GLOBAL T*)t.p;
   const ADZ_GLOBAL T* mng = (const ADZ_GLOBAL T*)t.mn;
   const ADZ_GLOBAL T* mxg = (const ADZ_GLOBAL T*)t.mx;
+  T mv[4], av[4], bv[4], lv[4], nv[4], xv[4];
+  double mk[4], pkv[4];
+  const bool need_lin = PASS == P_W || (PASS == P_CORR && !cyc);
+  const bool need_pk = need_lin && t.weight == 3;
+  const bool need_mm = PASS == P_CORR && !cyc && A.min_max;
+  // the conditions are uniform, so each array's loads of a group sit in one block and
+  // merge into one vector access
+  auto load4 = [&](long q, long q2) {
+#pragma unroll
+    for (int u = 0; u < 4; u++) { mk[u] = maskg[q2 + u]; mv[u] = mg[q + u]; av[u] = air0[q + u]; }
+    if (PASS == P_PM) {
+#pragma unroll
+      for (int u = 0; u < 4; u++) { bv[u] = air1[q + u]; lv[u] = pg[q + u]; }
+    }
+    if (need_lin) {
+#pragma unroll
+      for (int u = 0; u < 4; u++) lv[u] = lin[q + u];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++) pkv[u] = 0.0;
+    if (need_pk) {
+#pragma unroll
+      for (int u = 0; u < 4; u++) pkv[u] = (double)pk[q + u];
+    }
+    if (need_mm) {
+#pragma unroll
+      for (int u = 0; u < 4; u++) { nv[u] = mng[q + u]; xv[u] = mxg[q + u]; }
+    }
+  };
+  auto load = [&](long q, long q2, int u) {
+    mk[u] = maskg[q2];
+    mv[u] = mg[q];
+    av[u] = air0[q];
+    if (PASS == P_PM) { bv[u] = air1[q]; lv[u] = pg[q]; }
+    if (need_lin) lv[u] = lin[q];
+    pkv[u] = need_pk ? (double)pk[q] : 0.0;
+    if (need_mm) { nv[u] = mng[q]; xv[u] = mxg[q]; }
+  };
+  auto step = [&](long q, int u) {
+    T m = mv[u];
+    if (PASS == P_PM) {
+      dd_add(&s0, mass_term<T>(lv[u], bv[u], mk[u]));
+      dd_add(&s1, mass_term<T>(m, av[u], mk[u]));
+    } else if (PASS == P_W) {
+      T w = bc_weight<T>(m, lv[u], pkv[u], t, sd);
+      dd_add(&s0, mass_term<T>(w, av[u], mk[u]));
+    } else if (PASS == P_CORR) {
+      if (!cyc) {
+        T w = bc_weight<T>(m, lv[u], pkv[u], t, sd);
+        double corr = lambda * (double)w;
+        T bc = (T)((double)m - corr);
+        if (A.min_max) {
+          if (corr > 0.0 && bc < nv[u]) bc = nv[u];
+          if (corr < 0.0 && bc > xv[u]) bc = xv[u];
+        }
+        m = bc;
+        mg[q] = m;
+      }
+      dd_add(&s0, mass_term<T>(m, av[u], mk[u]));
+    } else {  // P_PROP
+      if (!cyc) {
+        m = (T)(prop * (double)m);
+        mg[q] = m;
+      }
+      dd_add(&s0, mass_term<T>(m, av[u], mk[u]));
+    }
+  };
   for (int r = r0; r < r1; r++) {
     const int j = A.j0 + r % A.ny, k = A.k0 + r / A.ny;
     const long q2r = (long)(A.i0 - 1) + (long)A.nxa * (j - 1);
     const long qr = q2r + (long)A.nxa * A.nya * (k - 1);
-    for (int ib = 0; ib < A.nx; ib += kU * kBcBlock) {
-      T mv[kU], av[kU], bv[kU], lv[kU], nv[kU], xv[kU];
-      double mk[kU], pkv[kU];
+    for (int ib = 0; ib < A.nx; ib += 4 * kBcBlock) {
+      const int ii0 = ib + 4 * (int)threadIdx.x;
+      const int nvalid = A.nx - ii0;
+      if (nvalid >= 4) {  // whole group: straight-line code, merged into vector accesses
+        load4(qr + ii0, q2r + ii0);
 #pragma unroll
-      for (int u = 0; u < kU; u++) {
-        const int ii = ib + u * kBcBlock + (int)threadIdx.x;
-        const bool in = ii < A.nx;
-        const long q = qr + (in ? ii : 0), q2 = q2r + (in ? ii : 0);
-        mk[u] = in ? maskg[q2] : 0.0;
-        mv[u] = mg[q];
-        av[u] = air0[q];
-        if (PASS == P_PM) { bv[u] = air1[q]; lv[u] = pg[q]; }
-        if (PASS == P_W || (PASS == P_CORR && !cyc)) {
-          lv[u] = lin[q];
-          pkv[u] = (t.weight == 3) ? (double)pk[q] : 0.0;
-        }
-        if (PASS == P_CORR && !cyc && A.min_max) { nv[u] = mng[q]; xv[u] = mxg[q]; }
-      }
-#pragma unroll
-      for (int u = 0; u < kU; u++) {
-        const int ii = ib + u * kBcBlock + (int)threadIdx.x;
-        if (ii >= A.nx) break;
-        ADZ_GLOBAL T* mp = mg + qr + ii;
-        T m = mv[u];
-        if (PASS == P_PM) {
-          dd_add(&s0, mass_term<T>(lv[u], bv[u], mk[u]));
-          dd_add(&s1, mass_term<T>(m, av[u], mk[u]));
-        } else if (PASS == P_W) {
-          T w = bc_weight<T>(m, lv[u], pkv[u], t, sd);
-          dd_add(&s0, mass_term<T>(w, av[u], mk[u]));
-        } else if (PASS == P_CORR) {
-          if (!cyc) {
-            T w = bc_weight<T>(m, lv[u], pkv[u], t, sd);
-            double corr = lambda * (double)w;
-            T bc = (T)((double)m - corr);
-            if (A.min_max) {
-              if (corr > 0.0 && bc < nv[u]) bc = nv[u];
-              if (corr < 0.0 && bc > xv[u]) bc = xv[u];
-            }
-            m = bc;
-            *mp = m;
-          }
-          dd_add(&s0, mass_term<T>(m, av[u], mk[u]));
-        } else {  // P_PROP
-          if (!cyc) {
-            m = (T)(prop * (double)m);
-            *mp = m;
-          }
-          dd_add(&s0, mass_term<T>(m, av[u], mk[u]));
-        }
+        for (int u = 0; u < 4; u++) step(qr + ii0 + u, u);
+      } else if (nvalid > 0) {  // the row's tail
+        for (int u = 0; u < nvalid; u++) load(qr + ii0 + u, q2r + ii0 + u, u);
+        for (int u = 0; u < nvalid; u++) step(qr + ii0 + u, u);
       }
     }
   }
