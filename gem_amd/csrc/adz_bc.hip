@@ -1,12 +1,15 @@
 // Bermejo-Conde mass fixer on gfx950 (src/gemdyn/src/base/Bermejo_Conde.F90:18-320, with
 // the global FP64 sums of mass_tr_PM.F90:75-175 and mass_tr_X.F90:66-111).
 //
-// The fixer needs four global sums per tracer, each one gating the next elementwise pass:
+// The fixer needs four global sums per tracer, each one gating the next elementwise step:
 //   1. masses of p (time P) and m (time M)        -> deficit D = M_m - (M_p - eps)
 //   2. mass of the weights w(m - lin, sign D)     -> lambda = D / W (skip if W is ~0)
 //   3. corrected m (min/max reset) and its mass  -> proportion = M_target / M_corrected
 //   4. proportional fix of m and its mass        -> new carried deficit eps
-// Every pass runs all tracers in one launch (grid.y = tracer).  Sums are deterministic:
+// Sums 1 and 2 share one pass: a weight is nonzero only where sign(m - lin) = sign(D), so
+// the pass accumulates the weight mass of both signs and the finishing step keeps the one
+// sign(D) selects (negation commutes with every rounding involved, so W is the same value
+// a separate pass would give).  Every pass runs all tracers in one launch.  Sums are deterministic:
 // each lane accumulates a fixed set of elements in double-double (TwoSum), blocks reduce
 // in a fixed tree, and a finalising kernel adds the block partials in index order; the
 // result is the exactly rounded sum except in astronomically rare ties, and independent of
@@ -20,6 +23,9 @@
 
 namespace adz {
 
+#ifndef ADZ_BC_UNROLL
+#define ADZ_BC_UNROLL 4  // elements gathered per lane before accumulating
+#endif
 constexpr int kBcBlock = 256;
 constexpr int kSc = 16;  // per-tracer scalars
 enum { SC_MP = 0, SC_MM, SC_MTP, SC_D, SC_W, SC_LAMBDA, SC_CYCLE, SC_MBCR, SC_PROP, SC_MBC };
@@ -50,7 +56,7 @@ struct BcArgs {
   const double* mask;
   double* deficit;
   double* diag;
-  double* part;              // [ntr][nblk][4]
+  double* part;              // [ntr][nblk][8]
   double* sc;                // [ntr][kSc]
 };
 
@@ -105,6 +111,19 @@ __device__ __forceinline__ T bc_weight(T m, T lin, double pkps, const BcTr& t, d
   return w;
 }
 
+// |w| and the sign of m - lin for the fused first pass (w = sign * |w| when the signs match)
+template <class T>
+__device__ __forceinline__ T bc_weight_abs(T m, T lin, double pkps, const BcTr& t, bool* pos) {
+  T hml = m - lin;
+  double H = (double)hml;
+  *pos = !signbit(H);
+  double a = fabs(H);
+  double v = (t.pexp == 1.0) ? a : (t.pexp == 2.0 ? a * a : pow(a, t.pexp));
+  if (t.weight == 2) v = v * (double)(m > (T)0 ? m : (T)0);
+  if (t.weight == 3) v = v * pkps;
+  return (T)v;
+}
+
 template <class T>
 __device__ __forceinline__ double mass_term(T x, T air, double mask) {
   T prod = x * air;  // real*real is a `real` product in the Fortran sums
@@ -113,7 +132,7 @@ __device__ __forceinline__ double mass_term(T x, T air, double mask) {
 
 // Grid: x = tracer (fastest), y = block of scope rows, so the blocks of every tracer over
 // the same rows are dispatched together and share the air mass, mask and pkps through
-// the caches.  Each block owns whole rows (a fixed partition).
+// the caches.  Each block owns whole rows (a fixed partition), lanes stride along i.
 template <class T, int PASS>
 __global__ __launch_bounds__(kBcBlock) void k_bc_pass(BcArgs A) {
   const int n = blockIdx.x;
@@ -122,124 +141,103 @@ __global__ __launch_bounds__(kBcBlock) void k_bc_pass(BcArgs A) {
   const bool cyc = (PASS >= P_CORR) && sc[SC_CYCLE] != 0.0;
   const double sd = sgn1(sc[SC_D]);
   const double lambda = sc[SC_LAMBDA], prop = sc[SC_PROP];
-  DD s0{0.0, 0.0}, s1{0.0, 0.0};
+  DD s0{0.0, 0.0}, s1{0.0, 0.0}, s2{0.0, 0.0}, s3{0.0, 0.0};
   const int r0 = (int)((long)A.nrows * blockIdx.y / A.nblk), r1 = (int)((long)A.nrows * (blockIdx.y + 1) / A.nblk);
   const ADZ_GLOBAL T* air1 = (const ADZ_GLOBAL T*)A.airm1;
   const ADZ_GLOBAL T* air0 = (const ADZ_GLOBAL T*)A.airm0;
   const ADZ_GLOBAL double* maskg = (const ADZ_GLOBAL double*)A.mask;
   ADZ_GLOBAL T* mg = (ADZ_GLOBAL T*)t.m;
-  // Each lane takes 4 consecutive elements of the row per batch (16-byte loads and stores
-  // for FP32), gathers all their inputs, then accumulates them in order.
+  // Each lane gathers the inputs of kU elements (kU * 256 apart along the row) before it
+  // accumulates them in order, so a wave keeps kU loads per array in flight.
+  constexpr int kU = ADZ_BC_UNROLL;
   const ADZ_GLOBAL T* lin = (const ADZ_GLOBAL T*)t.lin;
   const ADZ_GLOBAL T* pk = (const ADZ_GLOBAL T*)A.pkps;
   const ADZ_GLOBAL T* pg = (const ADZ_GLOBAL T*)t.p;
   const ADZ_GLOBAL T* mng = (const ADZ_GLOBAL T*)t.mn;
   const ADZ_GLOBAL T* mxg = (const ADZ_GLOBAL T*)t.mx;
-  T mv[4], av[4], bv[4], lv[4], nv[4], xv[4];
-  double mk[4], pkv[4];
-  const bool need_lin = PASS == P_W || (PASS == P_CORR && !cyc);
-  const bool need_pk = need_lin && t.weight == 3;
-  const bool need_mm = PASS == P_CORR && !cyc && A.min_max;
-  // the conditions are uniform, so each array's loads of a group sit in one block and
-  // merge into one vector access
-  auto load4 = [&](long q, long q2) {
-#pragma unroll
-    for (int u = 0; u < 4; u++) { mk[u] = maskg[q2 + u]; mv[u] = mg[q + u]; av[u] = air0[q + u]; }
-    if (PASS == P_PM) {
-#pragma unroll
-      for (int u = 0; u < 4; u++) { bv[u] = air1[q + u]; lv[u] = pg[q + u]; }
-    }
-    if (need_lin) {
-#pragma unroll
-      for (int u = 0; u < 4; u++) lv[u] = lin[q + u];
-    }
-#pragma unroll
-    for (int u = 0; u < 4; u++) pkv[u] = 0.0;
-    if (need_pk) {
-#pragma unroll
-      for (int u = 0; u < 4; u++) pkv[u] = (double)pk[q + u];
-    }
-    if (need_mm) {
-#pragma unroll
-      for (int u = 0; u < 4; u++) { nv[u] = mng[q + u]; xv[u] = mxg[q + u]; }
-    }
-  };
-  auto load = [&](long q, long q2, int u) {
-    mk[u] = maskg[q2];
-    mv[u] = mg[q];
-    av[u] = air0[q];
-    if (PASS == P_PM) { bv[u] = air1[q]; lv[u] = pg[q]; }
-    if (need_lin) lv[u] = lin[q];
-    pkv[u] = need_pk ? (double)pk[q] : 0.0;
-    if (need_mm) { nv[u] = mng[q]; xv[u] = mxg[q]; }
-  };
-  auto step = [&](long q, int u) {
-    T m = mv[u];
-    if (PASS == P_PM) {
-      dd_add(&s0, mass_term<T>(lv[u], bv[u], mk[u]));
-      dd_add(&s1, mass_term<T>(m, av[u], mk[u]));
-    } else if (PASS == P_W) {
-      T w = bc_weight<T>(m, lv[u], pkv[u], t, sd);
-      dd_add(&s0, mass_term<T>(w, av[u], mk[u]));
-    } else if (PASS == P_CORR) {
-      if (!cyc) {
-        T w = bc_weight<T>(m, lv[u], pkv[u], t, sd);
-        double corr = lambda * (double)w;
-        T bc = (T)((double)m - corr);
-        if (A.min_max) {
-          if (corr > 0.0 && bc < nv[u]) bc = nv[u];
-          if (corr < 0.0 && bc > xv[u]) bc = xv[u];
-        }
-        m = bc;
-        mg[q] = m;
-      }
-      dd_add(&s0, mass_term<T>(m, av[u], mk[u]));
-    } else {  // P_PROP
-      if (!cyc) {
-        m = (T)(prop * (double)m);
-        mg[q] = m;
-      }
-      dd_add(&s0, mass_term<T>(m, av[u], mk[u]));
-    }
-  };
   for (int r = r0; r < r1; r++) {
     const int j = A.j0 + r % A.ny, k = A.k0 + r / A.ny;
     const long q2r = (long)(A.i0 - 1) + (long)A.nxa * (j - 1);
     const long qr = q2r + (long)A.nxa * A.nya * (k - 1);
-    for (int ib = 0; ib < A.nx; ib += 4 * kBcBlock) {
-      const int ii0 = ib + 4 * (int)threadIdx.x;
-      const int nvalid = A.nx - ii0;
-      if (nvalid >= 4) {  // whole group: straight-line code, merged into vector accesses
-        load4(qr + ii0, q2r + ii0);
+    for (int ib = 0; ib < A.nx; ib += kU * kBcBlock) {
+      T mv[kU], av[kU], bv[kU], lv[kU], nv[kU], xv[kU];
+      double mk[kU], pkv[kU];
 #pragma unroll
-        for (int u = 0; u < 4; u++) step(qr + ii0 + u, u);
-      } else if (nvalid > 0) {  // the row's tail
-        for (int u = 0; u < nvalid; u++) load(qr + ii0 + u, q2r + ii0 + u, u);
-        for (int u = 0; u < nvalid; u++) step(qr + ii0 + u, u);
+      for (int u = 0; u < kU; u++) {
+        const int ii = ib + u * kBcBlock + (int)threadIdx.x;
+        const bool in = ii < A.nx;
+        const long q = qr + (in ? ii : 0), q2 = q2r + (in ? ii : 0);
+        mk[u] = in ? maskg[q2] : 0.0;
+        mv[u] = mg[q];
+        av[u] = air0[q];
+        if (PASS == P_PM) { bv[u] = air1[q]; nv[u] = pg[q]; }
+        if (PASS == P_PM || PASS == P_W || (PASS == P_CORR && !cyc)) {
+          lv[u] = lin[q];
+          pkv[u] = (t.weight == 3) ? (double)pk[q] : 0.0;
+        }
+        if (PASS == P_CORR && !cyc && A.min_max) { nv[u] = mng[q]; xv[u] = mxg[q]; }
+      }
+#pragma unroll
+      for (int u = 0; u < kU; u++) {
+        const int ii = ib + u * kBcBlock + (int)threadIdx.x;
+        if (ii >= A.nx) break;
+        ADZ_GLOBAL T* mp = mg + qr + ii;
+        T m = mv[u];
+        if (PASS == P_PM) {
+          dd_add(&s0, mass_term<T>(nv[u], bv[u], mk[u]));
+          dd_add(&s1, mass_term<T>(m, av[u], mk[u]));
+          bool pos;
+          const double wt = mass_term<T>(bc_weight_abs<T>(m, lv[u], pkv[u], t, &pos), av[u], mk[u]);
+          dd_add(pos ? &s2 : &s3, wt);
+        } else if (PASS == P_W) {
+          T w = bc_weight<T>(m, lv[u], pkv[u], t, sd);
+          dd_add(&s0, mass_term<T>(w, av[u], mk[u]));
+        } else if (PASS == P_CORR) {
+          if (!cyc) {
+            T w = bc_weight<T>(m, lv[u], pkv[u], t, sd);
+            double corr = lambda * (double)w;
+            T bc = (T)((double)m - corr);
+            if (A.min_max) {
+              if (corr > 0.0 && bc < nv[u]) bc = nv[u];
+              if (corr < 0.0 && bc > xv[u]) bc = xv[u];
+            }
+            m = bc;
+            *mp = m;
+          }
+          dd_add(&s0, mass_term<T>(m, av[u], mk[u]));
+        } else {  // P_PROP
+          if (!cyc) {
+            m = (T)(prop * (double)m);
+            *mp = m;
+          }
+          dd_add(&s0, mass_term<T>(m, av[u], mk[u]));
+        }
       }
     }
   }
-  // fixed-order block tree in LDS
-  __shared__ double sh[4][kBcBlock];
-  sh[0][threadIdx.x] = s0.hi; sh[1][threadIdx.x] = s0.lo;
-  sh[2][threadIdx.x] = s1.hi; sh[3][threadIdx.x] = s1.lo;
+  // fixed-order block tree in LDS (P_PM carries four sums, the other passes one)
+  constexpr int NS = PASS == P_PM ? 4 : 1;
+  __shared__ double sh[2 * NS][kBcBlock];
+  const DD acc[4] = {s0, s1, s2, s3};
+#pragma unroll
+  for (int q = 0; q < NS; q++) { sh[2 * q][threadIdx.x] = acc[q].hi; sh[2 * q + 1][threadIdx.x] = acc[q].lo; }
   __syncthreads();
   for (int w = kBcBlock / 2; w > 0; w >>= 1) {
     if (threadIdx.x < w) {
-      DD a{sh[0][threadIdx.x], sh[1][threadIdx.x]}, b{sh[0][threadIdx.x + w], sh[1][threadIdx.x + w]};
-      DD c = dd_merge(a, b);
-      sh[0][threadIdx.x] = c.hi; sh[1][threadIdx.x] = c.lo;
-      if (PASS == P_PM) {
-        DD a2{sh[2][threadIdx.x], sh[3][threadIdx.x]}, b2{sh[2][threadIdx.x + w], sh[3][threadIdx.x + w]};
-        DD c2 = dd_merge(a2, b2);
-        sh[2][threadIdx.x] = c2.hi; sh[3][threadIdx.x] = c2.lo;
+#pragma unroll
+      for (int q = 0; q < NS; q++) {
+        DD a{sh[2 * q][threadIdx.x], sh[2 * q + 1][threadIdx.x]};
+        DD b{sh[2 * q][threadIdx.x + w], sh[2 * q + 1][threadIdx.x + w]};
+        DD c = dd_merge(a, b);
+        sh[2 * q][threadIdx.x] = c.hi; sh[2 * q + 1][threadIdx.x] = c.lo;
       }
     }
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    double* o = A.part + ((long)n * A.nblk + blockIdx.y) * 4;
-    o[0] = sh[0][0]; o[1] = sh[1][0]; o[2] = sh[2][0]; o[3] = sh[3][0];
+    double* o = A.part + ((long)n * A.nblk + blockIdx.y) * 8;
+#pragma unroll
+    for (int q = 0; q < 2 * NS; q++) o[q] = sh[q][0];
   }
 }
 
@@ -249,15 +247,16 @@ template <int PASS>
 __global__ void k_bc_fin(BcArgs A) {
   int n = blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= A.ntr) return;
-  DD s0{0.0, 0.0}, s1{0.0, 0.0};
-  const double* pp = A.part + (long)n * A.nblk * 4;
-  for (int b = 0; b < A.nblk; b++) {
-    s0 = dd_merge(s0, DD{pp[4 * b], pp[4 * b + 1]});
-    s1 = dd_merge(s1, DD{pp[4 * b + 2], pp[4 * b + 3]});
-  }
-  double sum0 = s0.hi + s0.lo + 0.0, sum1 = s1.hi + s1.lo + 0.0;
+  constexpr int NS = PASS == P_PM ? 4 : 1;
+  DD s[4] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
+  const double* pp = A.part + (long)n * A.nblk * 8;
+  for (int b = 0; b < A.nblk; b++)
+#pragma unroll
+    for (int q = 0; q < NS; q++) s[q] = dd_merge(s[q], DD{pp[8 * b + 2 * q], pp[8 * b + 2 * q + 1]});
+  double sum0 = s[0].hi + s[0].lo + 0.0;
   double* sc = A.sc + n * kSc;
   if (PASS == P_PM) {
+    double sum1 = s[1].hi + s[1].lo + 0.0;
     // Bermejo_Conde.F90:92-105 (mass fluxes are zero: Aranami flux is not applied)
     double mass_p = sum0 - A.deficit[n];
     double mass_bflux = 0.0 - 0.0;
@@ -266,6 +265,13 @@ __global__ void k_bc_fin(BcArgs A) {
     sc[SC_MTP] = mass_p + mass_bflux;
     sc[SC_D] = sum1 - sc[SC_MTP];
     sc[SC_CYCLE] = 0.0;
+    // weight mass of the sign of D (:112-172): the positive sum, or minus the magnitude
+    // sum (a sum of zeros is +0, as a separate pass over signed weights gives)
+    double wpos = s[2].hi + s[2].lo + 0.0, wneg = s[3].hi + s[3].lo + 0.0;
+    double W = sgn1(sc[SC_D]) > 0.0 ? wpos : (wneg == 0.0 ? 0.0 : -wneg);
+    sc[SC_W] = W;
+    if (d_almost_zero(W)) sc[SC_CYCLE] = 1.0;                   // :190-198
+    else sc[SC_LAMBDA] = sc[SC_D] / W;                          // :200
   } else if (PASS == P_W) {
     sc[SC_W] = sum0;
     if (d_almost_zero(sum0)) sc[SC_CYCLE] = 1.0;               // :190-198
@@ -326,7 +332,7 @@ int adz_bc_create(const adz_bc_tracer* tr, int ntr, const adz_bc_domain* dom, in
   if (nblk > a.nrows) nblk = a.nrows;
   if (nblk < 1) nblk = 1;
   a.nblk = (int)nblk;
-  size_t tb = sizeof(BcTr) * ntr, pb = sizeof(double) * 4 * ntr * nblk, sb = sizeof(double) * kSc * ntr;
+  size_t tb = sizeof(BcTr) * ntr, pb = sizeof(double) * 8 * ntr * nblk, sb = sizeof(double) * kSc * ntr;
   void* mem = nullptr;
   if (hipMalloc(&mem, tb + pb + sb) != hipSuccess) return fail(ADZ_ENOMEM, "adz_bc_create: hipMalloc");
   if (hipMemcpy(mem, h.data(), tb, hipMemcpyHostToDevice) != hipSuccess ||
@@ -357,8 +363,6 @@ int adz_bermejo_conde(adz_bc* bc, const void* airm1, const void* airm0, const vo
 #define ADZ_BC_PASSES(T)                                                       \
   hipLaunchKernelGGL((k_bc_pass<T, P_PM>), g, b, 0, st, a);                   \
   hipLaunchKernelGGL((k_bc_fin<P_PM>), gf, bf, 0, st, a);                     \
-  hipLaunchKernelGGL((k_bc_pass<T, P_W>), g, b, 0, st, a);                    \
-  hipLaunchKernelGGL((k_bc_fin<P_W>), gf, bf, 0, st, a);                      \
   hipLaunchKernelGGL((k_bc_pass<T, P_CORR>), g, b, 0, st, a);                 \
   hipLaunchKernelGGL((k_bc_fin<P_CORR>), gf, bf, 0, st, a);                   \
   if (a.min_max) {                                                            \
