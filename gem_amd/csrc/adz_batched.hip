@@ -184,16 +184,16 @@ __device__ __forceinline__ void per_lane(const DevItem& it, const Fields& F, int
   const long gni = it.lv.ni, gnij = it.lv.nij;
   if constexpr (std::is_same<TF, float>::value) {
     if (it.span) {
-      int bo[4][4];
+      // per-lane byte offsets of the 4 planes of row 0; rows r add r*ni, a uniform offset
+      // that goes in the instruction's scalar offset
+      int bo[4];
       if (ok) {
         const long o1 = x.zlinear ? 0 : gnij, o2 = o1 + gnij, o3 = o2 + o1;
         const long e = x.off - it.minoff;
 #pragma unroll
-        for (int r = 0; r < 4; r++)
-#pragma unroll
-          for (int z = 0; z < 4; z++)
-            bo[r][z] = (int)((e + r * gni + (z == 0 ? 0 : (z == 1 ? o1 : (z == 2 ? o2 : o3)))) * 4);
+        for (int z = 0; z < 4; z++) bo[z] = (int)((e + (z == 0 ? 0 : (z == 1 ? o1 : (z == 2 ? o2 : o3)))) * 4);
       }
+      const int rstride = (int)(gni * 4);
       for (int f = f0; f < f1; f++) {
         if (ok) {
           const __amdgpu_buffer_rsrc_t rs = field_rsrc((const float*)F.s(f) + it.minoff, it.span);
@@ -202,7 +202,7 @@ __device__ __forceinline__ void per_lane(const DevItem& it, const Fields& F, int
           for (int r = 0; r < 4; r++)
 #pragma unroll
             for (int z = 0; z < 4; z++)
-              v[r][z] = __builtin_bit_cast(f32x4, __builtin_amdgcn_raw_buffer_load_b128(rs, bo[r][z], 0, 0));
+              v[r][z] = __builtin_bit_cast(f32x4, __builtin_amdgcn_raw_buffer_load_b128(rs, bo[z], r * rstride, 0));
           eval_store_a<TF, MATH, POST>(F, f, RegStencil{v}, cw, o);
         } else if (valid) {
           store_nan<TF, MATH, POST>(F, f, o);
