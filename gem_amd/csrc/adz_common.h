@@ -184,20 +184,23 @@ __device__ __forceinline__ void tricub3_ref(const TF* __restrict__ f, const doub
                                             int zlinear, double* out) {
   const long o1 = zlinear ? 0 : nij3;
   const long o2 = o1 + nij3, o3 = o2 + o1;
+  // rows outer, the 12 interleaved values of a row inner: only one row's values and the
+  // 12 running sums are live at a time (each element keeps the same operation order)
   double dst[12];
 #pragma unroll
-  for (int e = 0; e < 12; e++) {
-    double ya = 0.0;
+  for (int r = 0; r < 4; r++) {
+    const TF* s = f + r * ni3;
+    double za[12];
 #pragma unroll
-    for (int r = 0; r < 4; r++) {
-      const TF* s = f + r * ni3;
-      double za = to_f64(s[e]) * c[8];
-      za = fma(to_f64(s[e + o1]), c[9], za);
-      za = fma(to_f64(s[e + o2]), c[10], za);
-      za = fma(to_f64(s[e + o3]), c[11], za);
-      ya = (r == 0) ? za * c[4] : fma(za, c[4 + r], ya);
-    }
-    dst[e] = ya;
+    for (int e = 0; e < 12; e++) za[e] = to_f64(s[e]) * c[8];
+#pragma unroll
+    for (int e = 0; e < 12; e++) za[e] = fma(to_f64(s[e + o1]), c[9], za[e]);
+#pragma unroll
+    for (int e = 0; e < 12; e++) za[e] = fma(to_f64(s[e + o2]), c[10], za[e]);
+#pragma unroll
+    for (int e = 0; e < 12; e++) za[e] = fma(to_f64(s[e + o3]), c[11], za[e]);
+#pragma unroll
+    for (int e = 0; e < 12; e++) dst[e] = (r == 0) ? za[e] * c[4] : fma(za[e], c[4 + r], dst[e]);
   }
 #pragma unroll
   for (int q = 0; q < 3; q++) {
