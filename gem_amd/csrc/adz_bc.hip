@@ -95,23 +95,14 @@ __device__ __forceinline__ bool d_almost_zero(double a) {
 __device__ __forceinline__ double sgn1(double b) { return signbit(b) ? -1.0 : 1.0; }
 
 // w of Bermejo_Conde.F90:112-172 for one element (exactly the stored `real` weight)
-// |H|^pexp: pexp 1 and 2 exactly; other exponents through pow (only in the GENP instances,
-// whose register use the common case does not pay for)
-template <bool GENP>
-__device__ __forceinline__ double bc_pow(double a, double p) {
-  if (p == 1.0) return a;
-  if (p == 2.0 || !GENP) return a * a;
-  return pow(a, p);
-}
-
-template <class T, bool GENP>
+template <class T>
 __device__ __forceinline__ T bc_weight(T m, T lin, double pkps, const BcTr& t, double sd) {
   T hml = m - lin;
   double H = (double)hml;
   T w = (T)0;
   if (sd == sgn1(H)) {
     double a = fabs(H);
-    double v = bc_pow<GENP>(a, t.pexp);
+    double v = (t.pexp == 1.0) ? a : (t.pexp == 2.0 ? a * a : pow(a, t.pexp));
     v = v * sd;
     if (t.weight == 2) v = v * (double)(m > (T)0 ? m : (T)0);
     if (t.weight == 3) v = v * pkps;
@@ -121,13 +112,13 @@ __device__ __forceinline__ T bc_weight(T m, T lin, double pkps, const BcTr& t, d
 }
 
 // |w| and the sign of m - lin for the fused first pass (w = sign * |w| when the signs match)
-template <class T, bool GENP>
+template <class T>
 __device__ __forceinline__ T bc_weight_abs(T m, T lin, double pkps, const BcTr& t, bool* pos) {
   T hml = m - lin;
   double H = (double)hml;
   *pos = !signbit(H);
   double a = fabs(H);
-  double v = bc_pow<GENP>(a, t.pexp);
+  double v = (t.pexp == 1.0) ? a : (t.pexp == 2.0 ? a * a : pow(a, t.pexp));
   if (t.weight == 2) v = v * (double)(m > (T)0 ? m : (T)0);
   if (t.weight == 3) v = v * pkps;
   return (T)v;
@@ -142,7 +133,7 @@ __device__ __forceinline__ double mass_term(T x, T air, double mask) {
 // Grid: x = tracer (fastest), y = block of scope rows, so the blocks of every tracer over
 // the same rows are dispatched together and share the air mass, mask and pkps through
 // the caches.  Each block owns whole rows (a fixed partition), lanes stride along i.
-template <class T, int PASS, bool GENP>
+template <class T, int PASS>
 __global__ __launch_bounds__(kBcBlock) void k_bc_pass(BcArgs A) {
   const int n = blockIdx.x;
   const BcTr t = A.tr[n];
@@ -196,14 +187,14 @@ __global__ __launch_bounds__(kBcBlock) void k_bc_pass(BcArgs A) {
           dd_add(&s0, mass_term<T>(nv[u], bv[u], mk[u]));
           dd_add(&s1, mass_term<T>(m, av[u], mk[u]));
           bool pos;
-          const double wt = mass_term<T>(bc_weight_abs<T, GENP>(m, lv[u], pkv[u], t, &pos), av[u], mk[u]);
+          const double wt = mass_term<T>(bc_weight_abs<T>(m, lv[u], pkv[u], t, &pos), av[u], mk[u]);
           dd_add(pos ? &s2 : &s3, wt);
         } else if (PASS == P_W) {
-          T w = bc_weight<T, GENP>(m, lv[u], pkv[u], t, sd);
+          T w = bc_weight<T>(m, lv[u], pkv[u], t, sd);
           dd_add(&s0, mass_term<T>(w, av[u], mk[u]));
         } else if (PASS == P_CORR) {
           if (!cyc) {
-            T w = bc_weight<T, GENP>(m, lv[u], pkv[u], t, sd);
+            T w = bc_weight<T>(m, lv[u], pkv[u], t, sd);
             double corr = lambda * (double)w;
             T bc = (T)((double)m - corr);
             if (A.min_max) {
@@ -305,7 +296,6 @@ using namespace adz;
 struct adz_bc {
   int tdtype;
   int has_w3;  // some tracer uses weight 3 (needs pkps)
-  int genp;    // some tracer's weight exponent is neither 1 nor 2 (needs pow)
   BcArgs a;
   void* mem;
 };
@@ -356,11 +346,7 @@ int adz_bc_create(const adz_bc_tracer* tr, int ntr, const adz_bc_domain* dom, in
   adz_bc* b = new adz_bc;
   b->tdtype = tdtype; b->a = a; b->mem = mem;
   b->has_w3 = 0;
-  b->genp = 0;
-  for (int n = 0; n < ntr; n++) {
-    b->has_w3 |= (tr[n].weight == 3);
-    b->genp |= (h[n].pexp != 1.0 && h[n].pexp != 2.0);
-  }
+  for (int n = 0; n < ntr; n++) b->has_w3 |= (tr[n].weight == 3);
   *out = b;
   return ADZ_OK;
 }
@@ -374,19 +360,19 @@ int adz_bermejo_conde(adz_bc* bc, const void* airm1, const void* airm0, const vo
   hipStream_t st = (hipStream_t)stream;
   dim3 g((unsigned)a.ntr, (unsigned)a.nblk), b(kBcBlock);
   dim3 gf((unsigned)((a.ntr + 63) / 64)), bf(64);
-#define ADZ_BC_PASSES(T, G)                                                    \
-  hipLaunchKernelGGL((k_bc_pass<T, P_PM, G>), g, b, 0, st, a);                \
+#define ADZ_BC_PASSES(T)                                                       \
+  hipLaunchKernelGGL((k_bc_pass<T, P_PM>), g, b, 0, st, a);                   \
   hipLaunchKernelGGL((k_bc_fin<P_PM>), gf, bf, 0, st, a);                     \
-  hipLaunchKernelGGL((k_bc_pass<T, P_CORR, G>), g, b, 0, st, a);              \
+  hipLaunchKernelGGL((k_bc_pass<T, P_CORR>), g, b, 0, st, a);                 \
   hipLaunchKernelGGL((k_bc_fin<P_CORR>), gf, bf, 0, st, a);                   \
   if (a.min_max) {                                                            \
-    hipLaunchKernelGGL((k_bc_pass<T, P_PROP, G>), g, b, 0, st, a);            \
+    hipLaunchKernelGGL((k_bc_pass<T, P_PROP>), g, b, 0, st, a);               \
     hipLaunchKernelGGL((k_bc_fin<P_PROP>), gf, bf, 0, st, a);                 \
   }
   if (bc->tdtype == ADZ_F32) {
-    if (bc->genp) { ADZ_BC_PASSES(float, true) } else { ADZ_BC_PASSES(float, false) }
+    ADZ_BC_PASSES(float)
   } else {
-    if (bc->genp) { ADZ_BC_PASSES(double, true) } else { ADZ_BC_PASSES(double, false) }
+    ADZ_BC_PASSES(double)
   }
 #undef ADZ_BC_PASSES
   ADZ_HIP_CHECK(hipGetLastError());
