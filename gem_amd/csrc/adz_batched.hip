@@ -76,11 +76,17 @@ struct ItemFields {  // field pointer tables of a plan item (device arrays in gl
   void* const* mx;
   typedef const void* cvp;
   typedef void* vp;
-  __device__ const void* s(int f) const { return ((const cvp ADZ_GLOBAL*)src)[f]; }
-  __device__ void* d(int f) const { return ((const vp ADZ_GLOBAL*)dst)[f]; }
-  __device__ void* l(int f) const { return ((const vp ADZ_GLOBAL*)lin)[f]; }
-  __device__ void* n(int f) const { return ((const vp ADZ_GLOBAL*)mn)[f]; }
-  __device__ void* x(int f) const { return ((const vp ADZ_GLOBAL*)mx)[f]; }
+  // The tables do not change during a launch: reading them through the constant address
+  // space lets the compiler use scalar loads for the (wave-uniform) field index, which
+  // keeps them off the vector-memory counter.
+#define ADZ_CONST __attribute__((address_space(4)))
+  __device__ const void* s(int f) const { return ((const cvp ADZ_CONST*)src)[f]; }
+  __device__ void* d(int f) const { return ((const vp ADZ_CONST*)dst)[f]; }
+  __device__ void* l(int f) const { return ((const vp ADZ_CONST*)lin)[f]; }
+  __device__ void* n(int f) const { return ((const vp ADZ_CONST*)mn)[f]; }
+  __device__ void* x(int f) const { return ((const vp ADZ_CONST*)mx)[f]; }
+  __device__ const void* sc(int f) const { return s(f); }
+#undef ADZ_CONST
 };
 
 // Evaluate one target from its stencil accessor and store the outputs at dst offset o.
@@ -152,7 +158,7 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t field_rsrc(const void* base, u
   unsigned long long a = (unsigned long long)base;
   unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)a), hi = __builtin_amdgcn_readfirstlane((unsigned)(a >> 32));
   void* p = (void*)(((unsigned long long)hi << 32) | lo);
-  return __builtin_amdgcn_make_buffer_rsrc(p, (short)0, (int)bytes, 0x00020000);
+  return __builtin_amdgcn_make_buffer_rsrc(p, (short)0, (int)__builtin_amdgcn_readfirstlane(bytes), 0x00020000);
 }
 
 template <class TF, int MATH, int POST, class Fields>
@@ -331,6 +337,235 @@ __device__ __forceinline__ void run_tile(const DevItem& it, const Fields& F, lon
 
 
 
+// ---------------------------------------------------------------------------------
+// Staging mode 3 (FP32 sources): a producer / consumer block of 5 wavefronts.
+//
+// Wave 0 only copies each field's box (the union of the tile's stencils) into a ring of
+// LDS slots with 16-byte buffer-to-LDS DMA, several fields ahead.  It issues no stores,
+// so its vector-memory counter counts its own DMA in order, and s_waitcnt vmcnt(k*R)
+// tells it when an older field has landed; it then publishes that field in an LDS flag.
+// Waves 1..4 (one target row each) wait for the flag, read their stencils from LDS,
+// evaluate, store, and return the slot with an LDS atomic.  Every wait is a bounded spin
+// (it gives up after about 2^22 sleeps instead of hanging), and one block barrier per
+// tile separates tiles.
+constexpr int kPCWaves = 5;
+constexpr int kPCThreads = kPCWaves * 64;
+constexpr int kRing = 12288;   // floats of slot ring per block (48 KB)
+constexpr int kMaxSlots = 8;
+constexpr int kPCDepth = 3;    // fields in flight per producer
+
+__device__ __forceinline__ void wait_vmcnt(int n) {
+  switch (n) {
+#define ADZ_VMW(k) case k: asm volatile("s_waitcnt vmcnt(" #k ")" ::: "memory"); break;
+    ADZ_VMW(0) ADZ_VMW(1) ADZ_VMW(2) ADZ_VMW(3) ADZ_VMW(4) ADZ_VMW(5) ADZ_VMW(6) ADZ_VMW(7)
+    ADZ_VMW(8) ADZ_VMW(9) ADZ_VMW(10) ADZ_VMW(11) ADZ_VMW(12) ADZ_VMW(13) ADZ_VMW(14) ADZ_VMW(15)
+    ADZ_VMW(16) ADZ_VMW(17) ADZ_VMW(18) ADZ_VMW(19) ADZ_VMW(20) ADZ_VMW(21) ADZ_VMW(22) ADZ_VMW(23)
+    ADZ_VMW(24) ADZ_VMW(25) ADZ_VMW(26) ADZ_VMW(27) ADZ_VMW(28) ADZ_VMW(29) ADZ_VMW(30) ADZ_VMW(31)
+    ADZ_VMW(32) ADZ_VMW(33) ADZ_VMW(34) ADZ_VMW(35) ADZ_VMW(36) ADZ_VMW(37) ADZ_VMW(38) ADZ_VMW(39)
+    ADZ_VMW(40) ADZ_VMW(41) ADZ_VMW(42) ADZ_VMW(43) ADZ_VMW(44) ADZ_VMW(45) ADZ_VMW(46) ADZ_VMW(47)
+    ADZ_VMW(48) ADZ_VMW(49) ADZ_VMW(50) ADZ_VMW(51) ADZ_VMW(52) ADZ_VMW(53) ADZ_VMW(54) ADZ_VMW(55)
+    ADZ_VMW(56) ADZ_VMW(57) ADZ_VMW(58) ADZ_VMW(59) ADZ_VMW(60) ADZ_VMW(61) ADZ_VMW(62)
+#undef ADZ_VMW
+    default: asm volatile("s_waitcnt vmcnt(63)" ::: "memory"); break;
+  }
+}
+
+// Bounded spin on an LDS word until *p >= v.  A wait that runs out (about 2^20 sleeps)
+// raises the block's abort flag, and every later wait of the block returns at once, so a
+// broken hand-off costs one timeout per block instead of one per wait.
+typedef volatile __attribute__((address_space(3))) int lds_flag;  // LDS words read with ds_read
+typedef __attribute__((address_space(3))) float lds_float;
+
+__device__ __forceinline__ void spin_ge(lds_flag* p, int v, lds_flag* abort_flag) {
+  if (*abort_flag) return;
+  for (int it = 0; it < (1 << 20); it++) {
+    if (*p >= v) return;
+    __builtin_amdgcn_s_sleep(1);
+  }
+  *abort_flag = 1;
+}
+
+// Publish a flag word.  Written in assembly: the compiler would otherwise wait for every
+// outstanding LDS DMA before an LDS store it cannot prove disjoint from the DMA targets,
+// which would serialise the producer's pipeline.  The caller has already waited for the
+// DMA this flag announces.
+__device__ __forceinline__ void lds_publish(lds_flag* p, int v) {
+  const unsigned a = (unsigned)(unsigned long long)p;
+  asm volatile("ds_write_b32 %0, %1" ::"v"(a), "v"(v) : "memory");
+}
+
+struct PCBox {  // the tile's staged box and ring geometry (uniform)
+  int x0, y0, z0, nyb, cpr, nch, r, slot, ns;
+};
+
+template <int MATH, int POST, class TP>
+__device__ __forceinline__ void pc_tile(const DevItem& it, const ItemFields& F, long ptile, int f0, int f1,
+                                        lds_float* ring, lds_flag* ready, lds_flag* freed, int* red,
+                                        lds_flag* abort_flag, int level_inner) {
+  const int wave = threadIdx.x / 64, lane = threadIdx.x & 63;
+  const bool consumer = wave > 0;
+  const long nib = (it.nx + kTI - 1) / kTI, njb = (it.ny + kTJ - 1) / kTJ;
+  int ib, jb, kk;
+  if (level_inner) {
+    kk = (int)(ptile % it.nz); ib = (int)((ptile / it.nz) % nib); jb = (int)(ptile / (it.nz * nib));
+  } else {
+    ib = (int)(ptile % nib); jb = (int)((ptile / nib) % njb); kk = (int)(ptile / (nib * njb));
+  }
+  const int ii = ib * kTI + lane, jj = jb * kTJ + (wave - 1);
+  const bool valid = consumer && ii < it.nx && jj < it.ny;
+  const int i = it.i0 + ii, j = it.j0 + jj, k = it.k0 + kk;
+  const long p = (long)ii + (long)it.nx * (jj + (long)it.ny * kk);
+  const long o = (long)(i - 1) + (long)it.dni * ((j - 1) + (long)it.dnj * (k - 1));
+  Index x;
+  bool ok = false;
+  if (valid) {
+    double px, py, pz;
+    load_pos<TP>(it.xyz, p, &px, &py, &pz);
+    ok = index_step(px, py, pz, it.lv, &x);
+  }
+  double cw[20];
+  if (ok) {
+    if (MATH == ADZ_MATH_REF) weights_ref(x, it.lv, cw);
+    else weights_f32(x, it.lv, (float*)cw);
+  }
+  // box extents over the block (the producer contributes nothing)
+  const int big = 1 << 30;
+  int e[6] = {ok ? x.ix - 1 : big, ok ? -(x.ix + 2) : big, ok ? x.iy - 1 : big, ok ? -(x.iy + 2) : big,
+              ok ? x.pz : big, ok ? -(x.pz + (x.zlinear ? 1 : 3)) : big};
+#pragma unroll
+  for (int q = 0; q < 6; q++)
+    for (int sft = 32; sft > 0; sft >>= 1) e[q] = min(e[q], __shfl_xor(e[q], sft, 64));
+  if (lane == 0)
+#pragma unroll
+    for (int q = 0; q < 6; q++) red[q * kPCWaves + wave] = e[q];
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < 6; q++) {
+    int v = red[q * kPCWaves];
+    for (int w = 1; w < kPCWaves; w++) v = min(v, red[q * kPCWaves + w]);
+    e[q] = __builtin_amdgcn_readfirstlane(v);  // block-uniform (and known so to the compiler)
+  }
+  if (threadIdx.x < kMaxSlots) { ready[threadIdx.x] = 0; freed[threadIdx.x] = 0; }
+  __syncthreads();  // everyone has read `red`; the flags are reset
+  const long gni = it.lv.ni, gnij = it.lv.nij;
+  // item values the DMA needs, read once and held in scalar registers
+  const long minoff = ((long)__builtin_amdgcn_readfirstlane((int)(it.minoff >> 32)) << 32) |
+                      (unsigned)__builtin_amdgcn_readfirstlane((int)it.minoff);
+  const unsigned span = __builtin_amdgcn_readfirstlane(it.span);
+  PCBox b;
+  b.x0 = e[0]; b.y0 = e[2]; b.z0 = e[4];
+  const int nxb = -e[1] - e[0] + 1, nyb = -e[3] - e[2] + 1, nzb = -e[5] - e[4] + 1;
+  b.nyb = nyb;
+  b.cpr = (nxb + 3 + 3) / 4;              // 16-byte chunks per row (room for the alignment shift)
+  b.nch = nyb * nzb * b.cpr;
+  b.r = (b.nch + 63) / 64;                // DMA instructions per field
+  b.slot = b.r * 256;                     // floats per slot (each instruction fills 1 KB)
+  b.ns = e[0] == big ? 0 : kRing / b.slot;
+  if (b.ns > kMaxSlots) b.ns = kMaxSlots;
+  const bool staged = e[0] != big && b.ns >= 2 && span && (gni & 3) == 0 && (gnij & 3) == 0 && b.r <= 21;
+  if (!staged) {  // per-lane loads for this tile; the producer has nothing to do
+    if (consumer) per_lane<float, MATH, POST>(it, F, f0, f1, x, ok, valid, cw, o);
+    __syncthreads();
+    return;
+  }
+  const int nf = f1 - f0;
+  const int depth = min(kPCDepth, b.ns);
+  if (!consumer) {
+    // ------------------------------------------------------------------ producer
+    const float rcpr = 1.0f / (float)b.cpr, rnyb = 1.0f / (float)nyb;
+    for (int q = 0; q < nf; q++) {
+      const int sl = q % b.ns;
+      if (q >= b.ns) spin_ge(&freed[sl], 4 * (q / b.ns), abort_flag);
+      const float* base = (const float*)F.sc(f0 + q);
+      const int sh = (int)((((unsigned long long)base >> 2) + (unsigned)(b.x0 - 1)) & 3);
+      const __amdgpu_buffer_rsrc_t rs = field_rsrc(base + minoff, span);
+      lds_float* dstslot = ring + sl * b.slot;
+      for (int kq = 0; kq < b.r; kq++) {
+        const int c = kq * 64 + lane;
+        if (c < b.nch) {
+          const int rr = (int)(((float)c + 0.5f) * rcpr);
+          const int cc = c - rr * b.cpr;
+          const int zz = (int)(((float)rr + 0.5f) * rnyb);
+          const int yy = rr - zz * nyb;
+          const long el = (long)(b.x0 - 1 - sh) + (long)(b.y0 - 1 + yy) * gni + (long)(b.z0 + zz) * gnij + 4 * cc -
+                          minoff;
+          __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, (__attribute__((address_space(3))) void*)(dstslot + kq * 256),
+                                                   16, (int)(el * 4), 0, 0, 0);
+        }
+      }
+      if (q >= depth - 1) {  // the field issued depth-1 fields ago has landed
+        wait_vmcnt((depth - 1) * b.r);
+        if (lane == 0) lds_publish(&ready[(q - depth + 1) % b.ns], q - depth + 2);
+      }
+    }
+    wait_vmcnt(0);
+    if (lane == 0)
+      for (int q = max(0, nf - depth + 1); q < nf; q++) lds_publish(&ready[q % b.ns], q + 1);
+  } else {
+    // ------------------------------------------------------------------ consumers
+    // stencil row (r, plane z) starts at LDS float ((iy-1-y0+r) + nyb*(pz-z0+zp)) * W +
+    // (ix-1-x0) + shift, W = 4*cpr
+    const int W = 4 * b.cpr;
+    int lrow[4] = {0, 0, 0, 0};
+    if (ok) {
+      const int zp1 = x.zlinear ? 0 : 1, zp2 = zp1 + 1, zp3 = zp2 + zp1;
+      const int l0 = ((x.iy - 1 - b.y0) + nyb * (x.pz - b.z0)) * W + (x.ix - 1 - b.x0);
+      lrow[0] = l0; lrow[1] = l0 + zp1 * nyb * W; lrow[2] = l0 + zp2 * nyb * W; lrow[3] = l0 + zp3 * nyb * W;
+    }
+    for (int q = 0; q < nf; q++) {
+      const int sl = q % b.ns;
+      spin_ge(&ready[sl], q + 1, abort_flag);
+      const float* base = (const float*)F.sc(f0 + q);
+      const int sh = (int)((((unsigned long long)base >> 2) + (unsigned)(b.x0 - 1)) & 3);
+      if (ok) {
+        const lds_float* slot = ring + sl * b.slot + sh;
+        auto at = [&](int r, int z, int c) -> float { return slot[lrow[z] + r * W + c]; };
+        eval_store_a<float, MATH, POST>(F, f0 + q, at, cw, o);
+      } else if (valid) {
+        store_nan<float, MATH, POST>(F, f0 + q, o);
+      }
+      asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");  // this wave's LDS reads are done
+      if (lane == 0) __hip_atomic_fetch_add((__attribute__((address_space(3))) int*)&freed[sl], 1,
+                                            __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+  }
+  __syncthreads();
+}
+
+template <class TP, int MATH, int POST>
+__global__ __launch_bounds__(kPCThreads) void k_plan_pc(PlanArgs A) {
+  __shared__ float ring[kRing];
+  __shared__ int ready[kMaxSlots];
+  __shared__ int freed[kMaxSlots];
+  __shared__ int red[6 * kPCWaves];
+  __shared__ int abort_flag;
+  if (threadIdx.x == 0) abort_flag = 0;
+  __syncthreads();
+  long lo = 0, hi = A.ntiles, step = gridDim.x, t0 = blockIdx.x;
+  if (A.xcd_chunks) {
+    int nb = gridDim.x / 8;
+    int xc = blockIdx.x % 8, l = blockIdx.x / 8;
+    lo = A.ntiles * xc / 8;
+    hi = A.ntiles * (xc + 1) / 8;
+    step = nb;
+    t0 = lo + l;
+  }
+  for (long t = t0; t < hi; t += step) {
+    int q = 0;
+    const DevItem* items = A.items;
+    while (q + 1 < A.nitems && items[q + 1].tile0 <= t) q++;
+    const DevItem& it = items[q];
+    long local = t - it.tile0;
+    long ptile, grp;
+    if (A.points_outer) { ptile = local / it.ngroups; grp = local % it.ngroups; }
+    else { grp = local / it.ptiles; ptile = local % it.ptiles; }
+    int f0 = (int)grp * A.g, f1 = min(it.nf, f0 + A.g);
+    ItemFields F{it.src, it.dst, it.lin, it.mn, it.mx};
+    pc_tile<MATH, POST, TP>(it, F, ptile, f0, f1, (lds_float*)ring, (lds_flag*)ready, (lds_flag*)freed, red,
+                            (lds_flag*)&abort_flag, A.level_inner);
+  }
+}
+
 // Persistent loop over the tile list of a plan.
 template <class TF, class TP, int MATH, int POST, int STAGE>
 __device__ __forceinline__ void plan_loop(const PlanArgs& A, TF* boxA, TF* boxB, int* red, int* rowoff) {
@@ -427,6 +662,16 @@ bool fill_item(DevItem* d, const adz_ztab* geom, const void* xyz, int i0, int in
 // select the kernel instance for (fdtype, pdtype, math, post)
 template <class TF, class TP, int M>
 void* pick_plan(int post, int stage) {
+  if constexpr (std::is_same<TF, float>::value) {
+    if (stage == 3) {
+      switch (post) {
+        case 0: return (void*)&k_plan_pc<TP, M, 0>;
+        case 1: return (void*)&k_plan_pc<TP, M, 1>;
+        case 2: return (void*)&k_plan_pc<TP, M, 2>;
+        default: return (void*)&k_plan_pc<TP, M, 3>;
+      }
+    }
+  }
   if (stage == 1) {
     switch (post) {
       case 0: return (void*)&k_plan<TF, TP, M, 0, 1>;
@@ -596,7 +841,8 @@ int adz_plan_execute(const adz_plan* p, adz_stream stream) {
   void* kern = pick<true>(p->fdtype, p->pdtype, p->math, p->post, p->tune.stage);
   PlanArgs a = p->args;
   void* args[] = {&a};
-  ADZ_HIP_CHECK(hipLaunchKernel(kern, dim3(p->grid), dim3(kTile), args, 0, (hipStream_t)stream));
+  const unsigned nthr = (p->tune.stage == 3 && p->fdtype == ADZ_F32) ? kPCThreads : kTile;
+  ADZ_HIP_CHECK(hipLaunchKernel(kern, dim3(p->grid), dim3(nthr), args, 0, (hipStream_t)stream));
   return ADZ_OK;
 }
 
@@ -638,7 +884,7 @@ int adz_plan_autotune(const adz_item* items, int nitems, int fdtype, int math, i
   cur.fields_per_pass = std::min(cur.fields_per_pass, maxf);
   float cur_ms = 0.f;
   int rc = time_one(cur, &cur_ms);
-  const int stages[] = {1, 2}, fpps[] = {2, 4, 8, 16, 32, 64}, bpcs[] = {2, 4, 8}, flags[] = {0, 1};
+  const int stages[] = {1, 2, 3}, fpps[] = {2, 4, 8, 16, 32, 64}, bpcs[] = {2, 4, 8}, flags[] = {0, 1};
   // coordinates: 0 staging, 1 tile order, 2 fields per pass, 3 blocks per CU, 4 loop order,
   // 5 XCD chunking; the tile order is searched again after fields per pass (they interact
   // through the L2 working set)
@@ -646,7 +892,7 @@ int adz_plan_autotune(const adz_item* items, int nitems, int fdtype, int math, i
   for (int ci = 0; ci < 7 && rc == ADZ_OK; ci++) {
     const int coord = coords[ci];
     const int* vals = coord == 0 ? stages : coord == 2 ? fpps : coord == 3 ? bpcs : flags;
-    int nv = coord == 0 ? 2 : coord == 2 ? 6 : coord == 3 ? 3 : 2;
+    int nv = coord == 0 ? (fdtype == ADZ_F32 ? 3 : 2) : coord == 2 ? 6 : coord == 3 ? 3 : 2;
     for (int v = 0; v < nv && rc == ADZ_OK; v++) {
       adz_tune t = cur;
       int* slot = coord == 0 ? &t.stage : coord == 1 ? &t.level_inner : coord == 2 ? &t.fields_per_pass

@@ -209,7 +209,10 @@ typedef struct adz_tune {
                           over all points per field group                               */
   int blocks_per_cu;   /* persistent grid = blocks_per_cu * CUs                         */
   int xcd_chunks;      /* 1: each XCD takes a contiguous range of tiles                 */
-  int stage;           /* 1: stage each tile's source box in LDS; 2: per-lane loads     */
+  int stage;           /* 1: stage each tile's source box in LDS; 2: per-lane loads;
+                          3 (FP32 sources): one producer wave per block streams the
+                          boxes of several fields ahead into LDS (DMA), four consumer
+                          waves evaluate from LDS                                       */
   int level_inner;     /* 1: consecutive tiles walk the levels of one column (a source
                           plane feeds 4 levels: vertical reuse in L2); 0: i, j, then k   */
 } adz_tune;
