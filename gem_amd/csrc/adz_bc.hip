@@ -175,7 +175,12 @@ __global__ __launch_bounds__(kBcBlock) void k_bc_pass(BcArgs A) {
           lv[u] = lin[q];
           pkv[u] = (t.weight == 3) ? (double)pk[q] : 0.0;
         }
-        if (PASS == P_CORR && !cyc && A.min_max) { nv[u] = mng[q]; xv[u] = mxg[q]; }
+        // the correction lambda*w has the sign of D wherever it is nonzero (w and W both
+        // carry sign(D)), so only the bound on that side can trigger the reset
+        if (PASS == P_CORR && !cyc && A.min_max) {
+          if (sd > 0.0) { nv[u] = mng[q]; xv[u] = (T)0; }
+          else { xv[u] = mxg[q]; nv[u] = (T)0; }
+        }
       }
 #pragma unroll
       for (int u = 0; u < kU; u++) {
@@ -198,8 +203,8 @@ __global__ __launch_bounds__(kBcBlock) void k_bc_pass(BcArgs A) {
             double corr = lambda * (double)w;
             T bc = (T)((double)m - corr);
             if (A.min_max) {
-              if (corr > 0.0 && bc < nv[u]) bc = nv[u];
-              if (corr < 0.0 && bc > xv[u]) bc = xv[u];
+              if (sd > 0.0) { if (corr > 0.0 && bc < nv[u]) bc = nv[u]; }
+              else if (corr < 0.0 && bc > xv[u]) bc = xv[u];
             }
             m = bc;
             *mp = m;
