@@ -4,9 +4,11 @@
 Workload (default `c5`): 2048x1024x84 single-tile LAM grid, halo 5, dt = 450 s, winds
 with Courant numbers up to 6 (x, y) and 2 (z); 64 interpolated fields: u at the u-grid
 positions, v at the v-grid positions, T, q and 60 tracers at the thermodynamic positions;
-the Bermejo-Conde fixer on the 60 tracers.  One step = trajectory (adz_traject, FP64
-positions) + batched FP32 interpolation (persistent plans) + Bermejo-Conde, captured in
-a hipGraph and replayed.  Inputs are synthetic and already resident in HBM.
+the Bermejo-Conde fixer on the 60 tracers.  One step = trajectory (adz_traject: FP64
+iteration arithmetic and FP64 position history; positions and derived winds stored as the
+reference's real arrays by default, --traj-storage f64 for FP64 throughout) + batched FP32
+interpolation (persistent plans) + Bermejo-Conde, captured in a hipGraph and replayed.
+Inputs are synthetic and already resident in HBM.
 
 Printed JSON line: value = interpolated Gpoints/s of the whole step (points x fields /
 step time).  Beside it: the interpolation-kernel-only rate and HBM GB/s, the plain FP64
@@ -47,6 +49,9 @@ def parse():
     p.add_argument("--cpu-seconds", type=float, default=12.0)
     p.add_argument("--profile-kernel", action="store_true",
                    help="only build the workload and run the dominant interpolation kernel 5 times (rocprofv3)")
+    p.add_argument("--traj-storage", choices=("f32", "f64"), default="f32",
+                   help="storage of the trajectory state: f32 = the reference's real arrays (FP64 iteration "
+                        "arithmetic and FP64 position history), f64 = FP64 throughout (timed beside it either way)")
     p.add_argument("--retune", action="store_true",
                    help="run adz_plan_autotune instead of using the tiles it found before (gem_amd/tuned_tiles.json)")
     return p.parse_args()
@@ -77,7 +82,8 @@ def main():
     t_setup = time.time()
 
     # ------------------------------------------------------------------ state
-    ctx = A.AdzContext(g, L.ADZ_F64, itraj=cfg.niter)
+    pos_dt = L.ADZ_F32 if args.traj_storage == "f32" else L.ADZ_F64
+    ctx = A.AdzContext(g, pos_dt, itraj=cfg.niter)
     info = ctx.info
     w = {k: v.to(dev) for k, v in S.winds(cfg, g, device=dev).items()}
     ext_shape = (cfg.nk, cfg.nj + 2 * cfg.halo, cfg.ni + 2 * cfg.halo)
@@ -243,8 +249,16 @@ def main():
     t_plan0 = timed(plan0.execute, 5) if plan0 else float("nan")
     t_bc = sum(timed(lambda: bc(airm1, airm0, mask, deficit[i[0] - 4:i[-1] - 3], pkps=pkps), 2)
                for i, _, bc in groups)
-    log(f"phases: traject {t_traj:.2f} ms, interpolation {t_interp:.2f} ms (tracer-group kernel {t_plan0:.2f} ms), "
-        f"Bermejo-Conde {t_bc:.2f} ms")
+    # the trajectory with the other storage, beside it
+    other = L.ADZ_F64 if pos_dt == L.ADZ_F32 else L.ADZ_F32
+    ctx2 = A.AdzContext(g, other, itraj=cfg.niter)
+    ctx2.adz_inittraj()
+    traj2 = lambda: ctx2.adz_traject(cfg.dt, w["uu_ext"], w["vv_ext"], w["ww_ext"], w["ut0"], w["vt0"], w["zdt0"])
+    traj2()
+    t_traj2 = timed(traj2, 2)
+    ctx2.free()
+    log(f"phases: traject {t_traj:.2f} ms ({args.traj_storage} storage; {t_traj2:.2f} ms with the other), "
+        f"interpolation {t_interp:.2f} ms (tracer-group kernel {t_plan0:.2f} ms), Bermejo-Conde {t_bc:.2f} ms")
 
     # algorithmic bytes of the tracer-group kernel (one launch): sources + targets + lin/min/max
     # (3 extra FP32 outputs) + FP64 positions once
@@ -327,12 +341,15 @@ def main():
         "dtype": "f32",
         "data": "synthetic (seeded analytic winds, sinusoid and plume fields, generated in HBM)",
         "config": {"workload": f"{cfg.name}: {cfg.ni}x{cfg.nj}x{cfg.nk} LAM tile, halo {cfg.halo}, dt {cfg.dt:g} s, "
-                               f"{nf} fields (u, v, T, q + {ntr} tracers), FP32 storage, FP64 positions, "
+                               f"{nf} fields (u, v, T, q + {ntr} tracers), FP32 storage, FP64 trajectory arithmetic "
+                               f"with {args.traj_storage.upper()} position storage, "
                                f"Bermejo-Conde on {ntr} tracers, one step = trajectory + interpolation + fixer "
                                f"in one hipGraph",
                    "points_x_fields_per_step": pts_total, "bc_group": nb,
                    "tiles": {"uvtq": tuneA, "tracer_groups": tuneB}},
-        "phases_ms": {"traject": round(t_traj, 3), "interpolation": round(t_interp, 3),
+        "phases_ms": {"traject": round(t_traj, 3),
+                      "traject_" + ("f64" if args.traj_storage == "f32" else "f32") + "_storage": round(t_traj2, 3),
+                      "interpolation": round(t_interp, 3),
                       "bermejo_conde": round(t_bc, 3), "eager_step": round(eager_ms, 3)},
         "interp_gpoints_per_s": round(interp_gpts, 3),
         "interp_alg_GBps": round(alg_interp / (t_interp * 1e-3) / 1e9, 1),
@@ -362,7 +379,7 @@ def run_c4(args, np, torch, L, A, S, dev):
     state = []
     for cfg in subs:
         g = S.grid_desc(cfg)
-        ctx = A.AdzContext(g, L.ADZ_F64, itraj=cfg.niter)
+        ctx = A.AdzContext(g, L.ADZ_F32 if args.traj_storage == "f32" else L.ADZ_F64, itraj=cfg.niter)
         w = {k: v.to(dev) for k, v in S.winds(cfg, g, device=dev).items()}
         mod = (cfg.nk, cfg.nj + 2 * MH, cfg.ni + 2 * MH)
         rng = np.random.default_rng(cfg.seed)
@@ -481,7 +498,8 @@ def run_c4(args, np, torch, L, A, S, dev):
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic (seeded per-subdomain winds, lognormal front tracers, generated in HBM)",
         "config": {"workload": f"c4: 16 subdomains 64x48x84..512x384x84, {ntr} FP32 tracers each, one persistent "
-                               f"launch per step, {nst} steps (trajectories + launch) per hipGraph",
+                               f"launch per step, {nst} steps (trajectories + launch) per hipGraph, "
+                               f"{args.traj_storage.upper()} trajectory storage",
                    "points_x_fields_per_step": pf, "tune": tune},
         "phases_ms": {"traject_16": round(t_traj, 3), "persistent_launch": round(t_plan, 3),
                       "persistent_launch_default_tiles": round(t_plan_default, 3),
