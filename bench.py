@@ -359,7 +359,8 @@ def main():
         "error_vs_fp64": {"worst_max_rel": worst_max, "worst_rms_rel": worst_rms, "per_field_max_rms": errs},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK, 4), "traffic": traffic,
-                     "kernel": f"k_plan<float,double,F32,post> tracer group ({ng} fields), "
+                     "kernel": f"k_plan<float,{'float' if pos_dt == L.ADZ_F32 else 'double'},F32,post> tracer group "
+                               f"({ng} fields), "
                                f"{t_plan0:.3f} ms/launch, {alg_plan0 / 1e9:.2f} GB algorithmic"},
         "cpu_baseline": cpu,
     }
