@@ -45,6 +45,8 @@ def parse():
     p.add_argument("--bc-group", type=int, default=20, help="tracers per Bermejo-Conde group")
     p.add_argument("--fp64-fields", type=int, default=16, help="fields in the plain FP64 timing leg")
     p.add_argument("--err-fields", type=int, default=0, help="fields checked against FP64 (0 = all)")
+    p.add_argument("--no-lam-flux", action="store_true",
+                   help="c5: leave out the LAM boundary flux of the fixer (Adz_BC_LAM_flux = 0)")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--cpu-seconds", type=float, default=12.0)
     p.add_argument("--profile-kernel", action="store_true",
@@ -121,6 +123,8 @@ def main():
     airm1, airm0, pkps = airm1.reshape(-1).contiguous(), airm0.reshape(-1).contiguous(), pkps.reshape(-1).contiguous()
     mask = mask.reshape(-1).contiguous()
     deficit = torch.zeros(max(ntr, 1), dtype=torch.float64, device=dev)
+    fmass = torch.zeros(2 * max(ntr, 1), dtype=torch.float64, device=dev)  # (mass_fo, mass_fi) per tracer
+    lam_flux = not args.no_lam_flux and g["bc_lam_flux"] == 1
 
     ctx.adz_inittraj()
     step_traj = lambda: ctx.adz_traject(cfg.dt, w["uu_ext"], w["vv_ext"], w["ww_ext"], w["ut0"], w["vt0"],
@@ -128,6 +132,8 @@ def main():
     step_traj()
     torch.cuda.synchronize()
     pmu, pmv, pt = ctx.array(A.ADZ_A_PMU), ctx.array(A.ADZ_A_PMV), ctx.array(A.ADZ_A_PT)
+    pb = ctx.array(A.ADZ_A_PB)
+    npts_b = (info["inb"] - info["i0b"] + 1) * (info["jnb"] - info["j0b"] + 1) * cfg.nk
     boxes = {
         "u": ((info["i0u"], info["inu"], info["j0"], info["jn"], info["k0"], cfg.nk), pmu, ctx.ztab(False)),
         "v": ((info["i0"], info["in"], info["j0v"], info["jnv"], info["k0"], cfg.nk), pmv, ctx.ztab(False)),
@@ -170,6 +176,11 @@ def main():
                for k, q in enumerate(ids)]
         bc = A.BermejoConde(bcs, layout5, scope, mod_off, min_max=True)
         groups.append((ids, plan, bc))
+    # Aranami flux of every fixer tracer at Adz_pb (adz_interp_hlt.F90:91-96), one call for all
+    # tracers: the targets whose stencil crosses the core's edge are found once per step
+    fx = A.BcLamFlux([dict(src=src[q]) for q in range(4, nf)], layout5, (cfg.ni, cfg.nj),
+                     (info["i0b"], info["inb"], info["j0b"], info["jnb"]), scope, mod_off, ctx.ztab(True),
+                     pos_dtype=pos_dt) if lam_flux and ntr else None
     tuneB = tuneB if groups else None
     log(f"tiles ({'tuned now' if not cached else 'from gem_amd/tuned_tiles.json'}, {time.time() - t_tune:.1f}s): "
         f"u/v/T/q plan {tuneA}, tracer groups {tuneB}")
@@ -181,12 +192,19 @@ def main():
         for ids, plan, bc in groups:
             plan.execute()
 
+    def fixer(ids, bc):
+        sl = slice(ids[0] - 4, ids[-1] - 3)
+        fm = fmass[2 * (ids[0] - 4):2 * (ids[-1] - 3)] if fx else None
+        bc(airm1, airm0, mask, deficit[sl], pkps=pkps, flux=fm)
+
     def step():
         step_traj()
         planA.execute()
+        if fx:
+            fx(pb, airm0, mask, fmass)
         for ids, plan, bc in groups:
             plan.execute()
-            bc(airm1, airm0, mask, deficit[ids[0] - 4:ids[-1] - 3], pkps=pkps)
+            fixer(ids, bc)
 
     ev = lambda: torch.cuda.Event(enable_timing=True)
 
@@ -247,8 +265,9 @@ def main():
     t_interp = timed(interp, 3)
     ids0, plan0, bc0 = groups[0] if groups else (None, None, None)
     t_plan0 = timed(plan0.execute, 5) if plan0 else float("nan")
-    t_bc = sum(timed(lambda: bc(airm1, airm0, mask, deficit[i[0] - 4:i[-1] - 3], pkps=pkps), 2)
-               for i, _, bc in groups)
+    t_bc = sum(timed(lambda: fixer(i, bc), 2) for i, _, bc in groups)
+    t_flux = timed(lambda: fx(pb, airm0, mask, fmass), 3) if fx else 0.0
+    flux_active = fx.active() if fx else 0
     # the trajectory with the other storage, beside it
     other = L.ADZ_F64 if pos_dt == L.ADZ_F32 else L.ADZ_F32
     ctx2 = A.AdzContext(g, other, itraj=cfg.niter)
@@ -258,7 +277,8 @@ def main():
     t_traj2 = timed(traj2, 2)
     ctx2.free()
     log(f"phases: traject {t_traj:.2f} ms ({args.traj_storage} storage; {t_traj2:.2f} ms with the other), "
-        f"interpolation {t_interp:.2f} ms (tracer-group kernel {t_plan0:.2f} ms), Bermejo-Conde {t_bc:.2f} ms")
+        f"interpolation {t_interp:.2f} ms (tracer-group kernel {t_plan0:.2f} ms), Bermejo-Conde {t_bc:.2f} ms, "
+        f"LAM flux {t_flux:.2f} ms ({flux_active} of {npts_b} targets interpolated)")
 
     # algorithmic bytes of the tracer-group kernel (one launch): sources + targets + lin/min/max
     # (3 extra FP32 outputs) + FP64 positions once
@@ -343,14 +363,17 @@ def main():
         "config": {"workload": f"{cfg.name}: {cfg.ni}x{cfg.nj}x{cfg.nk} LAM tile, halo {cfg.halo}, dt {cfg.dt:g} s, "
                                f"{nf} fields (u, v, T, q + {ntr} tracers), FP32 storage, FP64 trajectory arithmetic "
                                f"with {args.traj_storage.upper()} position storage, "
-                               f"Bermejo-Conde on {ntr} tracers, one step = trajectory + interpolation + fixer "
-                               f"in one hipGraph",
+                               f"Bermejo-Conde on {ntr} tracers"
+                               f"{' with the LAM boundary flux (Aranami)' if lam_flux else ''}, "
+                               f"one step = trajectory + interpolation + fixer in one hipGraph",
                    "points_x_fields_per_step": pts_total, "bc_group": nb,
                    "tiles": {"uvtq": tuneA, "tracer_groups": tuneB}},
         "phases_ms": {"traject": round(t_traj, 3),
                       "traject_" + ("f64" if args.traj_storage == "f32" else "f32") + "_storage": round(t_traj2, 3),
                       "interpolation": round(t_interp, 3),
-                      "bermejo_conde": round(t_bc, 3), "eager_step": round(eager_ms, 3)},
+                      "bermejo_conde": round(t_bc, 3), "lam_flux": round(t_flux, 3),
+                      "eager_step": round(eager_ms, 3)},
+        "lam_flux_targets_interpolated": flux_active,
         "interp_gpoints_per_s": round(interp_gpts, 3),
         "interp_alg_GBps": round(alg_interp / (t_interp * 1e-3) / 1e9, 1),
         "fp64_plain_gpoints_per_s": round(plain_gpts, 3),

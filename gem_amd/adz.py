@@ -265,7 +265,12 @@ class Plan:
 L.extra_sigs({
     "adz_bc_create": ("i", ["vp", "i", "vp", "i", "vp"]),
     "adz_bermejo_conde": ("i", ["vp"] * 7 + ["vp"]),
+    "adz_bermejo_conde_lam": ("i", ["vp"] * 8 + ["vp"]),
     "adz_bc_free": ("i", ["vp"]),
+    "adz_flux_create": ("i", ["vp", "i", "vp", "vp", "i", "i", "vp"]),
+    "adz_bc_lam_flux": ("i", ["vp"] * 5 + ["vp"]),
+    "adz_flux_active": ("i", ["vp", "vp"]),
+    "adz_flux_free": ("i", ["vp"]),
 })
 
 
@@ -277,6 +282,13 @@ class AdzBcTracer(C.Structure):
 class AdzBcDomain(C.Structure):
     _fields_ = [(n, C.c_int) for n in ("minx", "maxx", "miny", "maxy", "nk", "i0", "in_", "j0", "jn", "k0",
                                        "min_max")] + [("pexp_list", C.c_float * 9)]
+
+
+def _check_types(dtype, **arrays):
+    """The library takes plain pointers: refuse arrays of another storage type."""
+    for name, a in arrays.items():
+        if a is not None and a.dtype != dtype:
+            raise TypeError(f"{name} is {a.dtype}, expected {dtype}")
 
 
 def bc_settings(mass):
@@ -310,17 +322,91 @@ class BermejoConde:
         h = C.c_void_p()
         L.check(L.lib().adz_bc_create(arr, len(tracers), C.byref(dom), tdt, C.byref(h)), "adz_bc_create")
         self.h, self.off, self.ntr = h, off, len(tracers)
+        self.dtype = tracers[0]["m"].dtype
         self.mask_off = (1 - minx) + (maxx - minx + 1) * (1 - miny)
 
-    def __call__(self, airm1, airm0, mask, deficit, pkps=None, diag=None):
-        L.check(L.lib().adz_bermejo_conde(self.h, _ptr_at(airm1, self.off), _ptr_at(airm0, self.off),
-                                          _ptr_at(pkps, self.off), _ptr_at(mask, self.mask_off),
-                                          deficit.data_ptr(), None if diag is None else diag.data_ptr(),
-                                          _stream()), "adz_bermejo_conde")
+    def __call__(self, airm1, airm0, mask, deficit, pkps=None, diag=None, flux=None):
+        """flux: FP64 tensor [2*ntr] of (mass_fo, mass_fi) from BcLamFlux (LAM Aranami flux)."""
+        _check_types(self.dtype, airm1=airm1, airm0=airm0, pkps=pkps)
+        _check_types(torch.float64, mask=mask, deficit=deficit, diag=diag, flux=flux)
+        if flux is not None and flux.numel() < 2 * self.ntr:
+            raise ValueError("flux needs 2 values per tracer")
+        args = (self.h, _ptr_at(airm1, self.off), _ptr_at(airm0, self.off), _ptr_at(pkps, self.off),
+                _ptr_at(mask, self.mask_off))
+        tail = (deficit.data_ptr(), None if diag is None else diag.data_ptr(), _stream())
+        if flux is None:
+            L.check(L.lib().adz_bermejo_conde(*args, *tail), "adz_bermejo_conde")
+        else:
+            L.check(L.lib().adz_bermejo_conde_lam(*args, flux.data_ptr(), *tail), "adz_bermejo_conde_lam")
 
     def free(self):
         if getattr(self, "h", None) is not None and self.h.value:
             L.lib().adz_bc_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------------- LAM boundary flux
+class AdzFluxTracer(C.Structure):
+    _fields_ = [("src", C.c_void_p), ("fo", C.c_void_p), ("fi", C.c_void_p)]
+
+
+class AdzFluxDomain(C.Structure):
+    _fields_ = [(n, C.c_int) for n in ("minx", "maxx", "miny", "maxy", "nk", "ni", "nj", "i0b", "inb", "j0b", "jnb",
+                                       "ic0", "ic1", "jc0", "jc1", "kc0")]
+
+
+class BcLamFlux:
+    """adz_BC_LAM_Aranami_hlt (adz_BC_LAM_Aranami_hlt.F90:18-236) for the fixer tracers,
+    with the flux masses of mass_tr_PM (mass_tr_PM.F90:101-127).
+
+    tracers: list of dicts with `src` (the tracer interpolated) and optionally `fo`, `fi`
+    (Adz_flux%fo / %fi, model layout, element (1,1,1) at offset `off`).  layout =
+    (minx, maxx, miny, maxy, nk); tile = (ni, nj); bbox = (i0b, inb, j0b, jnb) of the
+    Adz_pb targets; core = (i0, in, j0, jn, k0) of the fixer's scope.  geom: the thermo
+    descriptor (AdzContext.ztab(True)); pos_dtype: storage of the positions."""
+
+    def __init__(self, tracers, layout, tile, bbox, core, off, geom, pos_dtype=L.ADZ_F32):
+        minx, maxx, miny, maxy, nk = layout
+        arr = (AdzFluxTracer * len(tracers))()
+        tdt = None
+        for n, t in enumerate(tracers):
+            arr[n].src = _ptr_at(t["src"], off)
+            arr[n].fo = _ptr_at(t.get("fo"), off)
+            arr[n].fi = _ptr_at(t.get("fi"), off)
+            tdt = _DT[t["src"].dtype]
+        dom = AdzFluxDomain(minx, maxx, miny, maxy, nk, *tile, *bbox, *core)
+        h = C.c_void_p()
+        L.check(L.lib().adz_flux_create(arr, len(tracers), C.byref(dom), geom, tdt, pos_dtype, C.byref(h)),
+                "adz_flux_create")
+        self.h, self.off, self.ntr = h, off, len(tracers)
+        self.mask_off = (1 - minx) + (maxx - minx + 1) * (1 - miny)
+        self.dtype = tracers[0]["src"].dtype
+        self.pos_dtype = torch.float32 if pos_dtype == L.ADZ_F32 else torch.float64
+
+    def __call__(self, xyz, airm0, mask, mass):
+        """mass: FP64 device tensor [2*ntr] receiving (mass_fo, mass_fi) per tracer."""
+        _check_types(self.dtype, airm0=airm0)
+        _check_types(self.pos_dtype, xyz=xyz)
+        _check_types(torch.float64, mask=mask, mass=mass)
+        if mass.numel() < 2 * self.ntr:
+            raise ValueError("mass needs 2 values per tracer")
+        L.check(L.lib().adz_bc_lam_flux(self.h, xyz.data_ptr(), _ptr_at(airm0, self.off), _ptr_at(mask, self.mask_off),
+                                        mass.data_ptr(), _stream()), "adz_bc_lam_flux")
+
+    def active(self):
+        n = C.c_int()
+        L.check(L.lib().adz_flux_active(self.h, C.byref(n)), "adz_flux_active")
+        return n.value
+
+    def free(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            L.lib().adz_flux_free(self.h)
             self.h = None
 
     def __del__(self):

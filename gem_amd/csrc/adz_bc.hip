@@ -20,6 +20,7 @@
 #include <vector>
 
 #include "adz_common.h"
+#include "adz_dd.h"
 
 namespace adz {
 
@@ -56,33 +57,10 @@ struct BcArgs {
   const double* mask;
   double* deficit;
   double* diag;
+  const double* flux;        // [ntr][2] masses of FLUX_out, FLUX_in (LAM Aranami), or null
   double* part;              // [ntr][nblk][8]
   double* sc;                // [ntr][kSc]
 };
-
-struct DD {
-  double hi, lo;
-};
-__device__ __forceinline__ void two_sum(double a, double b, double* s, double* e) {
-  double x = a + b;
-  double bp = x - a;
-  *e = (a - (x - bp)) + (b - bp);
-  *s = x;
-}
-__device__ __forceinline__ void dd_add(DD* a, double t) {
-  double s, e;
-  two_sum(a->hi, t, &s, &e);
-  a->hi = s;
-  a->lo += e;
-}
-__device__ __forceinline__ DD dd_merge(DD a, DD b) {
-  double s, e;
-  two_sum(a.hi, b.hi, &s, &e);
-  e += a.lo + b.lo;
-  DD r;
-  two_sum(s, e, &r.hi, &r.lo);
-  return r;
-}
 
 // almost_zero.F90:16-36 (see the oracle's note: true for the patterns 0 and 1 only)
 __device__ __forceinline__ bool d_almost_zero(double a) {
@@ -122,12 +100,6 @@ __device__ __forceinline__ T bc_weight_abs(T m, T lin, double pkps, const BcTr& 
   if (t.weight == 2) v = v * (double)(m > (T)0 ? m : (T)0);
   if (t.weight == 3) v = v * pkps;
   return (T)v;
-}
-
-template <class T>
-__device__ __forceinline__ double mass_term(T x, T air, double mask) {
-  T prod = x * air;  // real*real is a `real` product in the Fortran sums
-  return (double)prod * mask;
 }
 
 // Grid: x = tracer (fastest), y = block of scope rows, so the blocks of every tracer over
@@ -262,9 +234,9 @@ __global__ void k_bc_fin(BcArgs A) {
   double* sc = A.sc + n * kSc;
   if (PASS == P_PM) {
     double sum1 = s[1].hi + s[1].lo + 0.0;
-    // Bermejo_Conde.F90:92-105 (mass fluxes are zero: Aranami flux is not applied)
+    // Bermejo_Conde.F90:92-105; the flux masses are zero without the Aranami flux
+    double mass_bflux = A.flux ? A.flux[2 * n + 1] - A.flux[2 * n] : 0.0 - 0.0;
     double mass_p = sum0 - A.deficit[n];
-    double mass_bflux = 0.0 - 0.0;
     sc[SC_MP] = mass_p;
     sc[SC_MM] = sum1;
     sc[SC_MTP] = mass_p + mass_bflux;
@@ -356,11 +328,16 @@ int adz_bc_create(const adz_bc_tracer* tr, int ntr, const adz_bc_domain* dom, in
   return ADZ_OK;
 }
 
-int adz_bermejo_conde(adz_bc* bc, const void* airm1, const void* airm0, const void* pkps, const double* mask,
-                      double* deficit, double* diag, adz_stream stream) {
+}  // extern "C"
+
+namespace {
+
+int bc_apply(adz_bc* bc, const void* airm1, const void* airm0, const void* pkps, const double* mask,
+             const double* flux, double* deficit, double* diag, adz_stream stream) {
   if (!bc || !airm1 || !airm0 || !mask || !deficit) return fail(ADZ_EINVAL, "adz_bermejo_conde: null argument");
   BcArgs a = bc->a;
   a.airm1 = airm1; a.airm0 = airm0; a.pkps = pkps; a.mask = mask; a.deficit = deficit; a.diag = diag;
+  a.flux = flux;
   if (bc->has_w3 && !pkps) return fail(ADZ_EINVAL, "adz_bermejo_conde: weight 3 needs pkps");
   hipStream_t st = (hipStream_t)stream;
   dim3 g((unsigned)a.ntr, (unsigned)a.nblk), b(kBcBlock);
@@ -382,6 +359,21 @@ int adz_bermejo_conde(adz_bc* bc, const void* airm1, const void* airm0, const vo
 #undef ADZ_BC_PASSES
   ADZ_HIP_CHECK(hipGetLastError());
   return ADZ_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int adz_bermejo_conde(adz_bc* bc, const void* airm1, const void* airm0, const void* pkps, const double* mask,
+                      double* deficit, double* diag, adz_stream stream) {
+  return bc_apply(bc, airm1, airm0, pkps, mask, nullptr, deficit, diag, stream);
+}
+
+int adz_bermejo_conde_lam(adz_bc* bc, const void* airm1, const void* airm0, const void* pkps, const double* mask,
+                          const double* flux_mass, double* deficit, double* diag, adz_stream stream) {
+  if (!flux_mass) return fail(ADZ_EINVAL, "adz_bermejo_conde_lam: null flux masses");
+  return bc_apply(bc, airm1, airm0, pkps, mask, flux_mass, deficit, diag, stream);
 }
 
 int adz_bc_free(adz_bc* bc) {

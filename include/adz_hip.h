@@ -274,7 +274,55 @@ int adz_bc_create(const adz_bc_tracer* tr, int ntr, const adz_bc_domain* dom, in
  * rank sequentially, mass_tr_X.F90:86-93).  No host synchronisation: capturable. */
 int adz_bermejo_conde(adz_bc* bc, const void* airm1, const void* airm0, const void* pkps, const double* mask,
                       double* deficit, double* diag, adz_stream stream);
+/* The same with the LAM boundary flux (Adz_BC_LAM_flux = 1): flux_mass is a DEVICE FP64
+ * array [2*ntr] of (mass_fo, mass_fi) per tracer, as adz_bc_lam_flux writes it; the target
+ * mass becomes (mass_p - deficit) + (mass_fi - mass_fo) (Bermejo_Conde.F90:92-105). */
+int adz_bermejo_conde_lam(adz_bc* bc, const void* airm1, const void* airm0, const void* pkps, const double* mask,
+                          const double* flux_mass, double* deficit, double* diag, adz_stream stream);
 int adz_bc_free(adz_bc* bc);
+
+/* ------------------------------------------------------------------------------------
+ * Level 4b: LAM boundary fluxes of the fixer, Aranami et al. (2015), Adz_BC_LAM_flux = 1:
+ * adz_BC_LAM_Aranami_hlt.F90:18-236 with the masks of adz_BC_LAM_setup.F90:18-92 and the
+ * flux masses of mass_tr_PM.F90:101-127.  For a tile that touches the four LAM edges
+ * (the one-tile case): mask_o is 1 on the core, mask_i is 1 on 1..ni x 1..nj x 1..nk
+ * outside it, both are 0 in the halo (where the reference's work arrays are never
+ * written and the clamped positions never reach).
+ * ----------------------------------------------------------------------------------*/
+
+/* One fixer tracer: DEVICE pointers to element (1,1,1) of model-layout arrays. */
+typedef struct adz_flux_tracer {
+  const void* src; /* the tracer interpolated (tracers_P, the source of adz_tricub_hlt) */
+  void* fo;        /* Adz_flux%fo (FLUX_out), or NULL: the flux arrays are not stored   */
+  void* fi;        /* Adz_flux%fi (FLUX_in), or NULL                                    */
+} adz_flux_tracer;
+
+typedef struct adz_flux_domain {
+  int minx, maxx, miny, maxy, nk; /* array layout (l_minx:l_maxx, l_miny:l_maxy, nk)            */
+  int ni, nj;                     /* the tile l_ni x l_nj                                       */
+  int i0b, inb, j0b, jnb;         /* targets Adz_i0b:Adz_inb x Adz_j0b:Adz_jnb x 1:nk (Adz_pb)  */
+  int ic0, ic1, jc0, jc1, kc0;    /* core 1+pil_w:l_ni-pil_e x 1+pil_s:l_nj-pil_n x Adz_k0t:nk  */
+} adz_flux_domain;
+
+typedef struct adz_flux adz_flux;
+/* Register ntr tracers (tdtype ADZ_F32 or ADZ_F64) and allocate the scratch (a class byte
+ * and a list slot per target, per-slice partial sums).  geom is the thermo-level
+ * descriptor (Adz_cpntr_t); the tracers are read on their model layout as adz_tricub_hlt
+ * reads non-extended sources.  pdtype is the storage type of the Adz_pb positions. */
+int adz_flux_create(const adz_flux_tracer* tr, int ntr, const adz_flux_domain* dom, const adz_ztab* geom, int tdtype,
+                    int pdtype, adz_flux** out);
+/* FLUX_out / FLUX_in of every registered tracer at the positions xyz (DEVICE Adz_pb), in
+ * the reference's arithmetic (ADZ_MATH_REF), stored when the tracers carry fo/fi arrays,
+ * and their masses with the air mass at time M (airm0, DEVICE, element (1,1,1)) and the
+ * DEVICE geomh_mask_8 (element (1,1)) into mass (DEVICE FP64 [2*ntr]: mass_fo, mass_fi),
+ * summed as adz_bermejo_conde sums.  Only the targets whose stencil crosses the core's
+ * boundary are interpolated; the others hold zeros.  Capturable (no host synchronisation). */
+int adz_bc_lam_flux(adz_flux* fx, const void* xyz, const void* airm0, const double* mask, double* mass,
+                    adz_stream stream);
+/* Number of targets interpolated by the last adz_bc_lam_flux (synchronises; diagnostics).
+ * No counterpart in the reference, which interpolates every target. */
+int adz_flux_active(const adz_flux* fx, int* count);
+int adz_flux_free(adz_flux* fx);
 
 #ifdef __cplusplus
 }

@@ -209,7 +209,7 @@ template <class T>
 static void bc_run(const int* dom, int ntr, const void* const* p, void* const* m, const void* const* lin,
                    const void* const* mn, const void* const* mx, const int* legacy, const int* weight,
                    const int* pexp_n, double* deficit, const void* airm1, const void* airm0, const void* pkps,
-                   const double* mask, const float* pexp_list, double* diag) {
+                   const double* mask, const float* pexp_list, double* diag, const double* flux) {
   BCDomain D{dom[0], dom[1], dom[2], dom[3], dom[4], dom[5], dom[6], dom[7], dom[8], dom[9], mask, dom[10],
              pexp_list, dom[11]};
   std::vector<BCTracer<T>> tr(ntr);
@@ -217,17 +217,20 @@ static void bc_run(const int* dom, int ntr, const void* const* p, void* const* m
     tr[n] = BCTracer<T>{(const T*)p[n], (T*)m[n], (const T*)lin[n], (const T*)mn[n], (const T*)mx[n],
                         legacy[n], weight[n], pexp_n[n], deficit[n]};
   std::vector<std::vector<T>> w(ntr);
-  bermejo_conde<T>(D, tr, (const T*)airm1, (const T*)airm0, (const T*)pkps, w, diag);
+  bermejo_conde<T>(D, tr, (const T*)airm1, (const T*)airm0, (const T*)pkps, w, diag, flux);
   for (int n = 0; n < ntr; n++) deficit[n] = tr[n].mass_deficit;
 }
 extern "C" void ora_bermejo_conde(int tdt, const int* dom, int ntr, const void* const* p, void* const* m,
                        const void* const* lin, const void* const* mn, const void* const* mx, const int* legacy,
                        const int* weight, const int* pexp_n, double* deficit, const void* airm1, const void* airm0,
-                       const void* pkps, const double* mask, const float* pexp_list, double* diag) {
+                       const void* pkps, const double* mask, const float* pexp_list, double* diag,
+                       const double* flux) {
   if (tdt == 1)
-    bc_run<float>(dom, ntr, p, m, lin, mn, mx, legacy, weight, pexp_n, deficit, airm1, airm0, pkps, mask, pexp_list, diag);
+    bc_run<float>(dom, ntr, p, m, lin, mn, mx, legacy, weight, pexp_n, deficit, airm1, airm0, pkps, mask, pexp_list,
+                  diag, flux);
   else
-    bc_run<double>(dom, ntr, p, m, lin, mn, mx, legacy, weight, pexp_n, deficit, airm1, airm0, pkps, mask, pexp_list, diag);
+    bc_run<double>(dom, ntr, p, m, lin, mn, mx, legacy, weight, pexp_n, deficit, airm1, airm0, pkps, mask, pexp_list,
+                   diag, flux);
 }
 extern "C" double ora_mass(int tdt, const int* dom, const void* x, const void* air, const double* mask) {
   BCDomain D{dom[0], dom[1], dom[2], dom[3], dom[4], dom[5], dom[6], dom[7], dom[8], dom[9], mask, 0, nullptr, dom[11]};

@@ -53,7 +53,7 @@ def lib():
         L.ora_almost_zero.argtypes = [d]
         L.ora_fsum.restype = d
         L.ora_fsum.argtypes = [vp, l]
-        L.ora_bermejo_conde.argtypes = [i, vp, i, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]
+        L.ora_bermejo_conde.argtypes = [i, vp, i, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]
         L.ora_mass.restype = d
         L.ora_mass.argtypes = [i, vp, vp, vp, vp]
         _lib = L
@@ -304,10 +304,11 @@ class Grid:
 
 
 def bermejo_conde(tracers, layout, scope, airm1, airm0, mask, deficit, pkps=None, min_max=True,
-                  pexp_list=(1.0,) * 9, sum_mode=1):
+                  pexp_list=(1.0,) * 9, sum_mode=1, flux=None):
     """Bermejo_Conde restatement on host arrays (array-start layout).  tracers: list of
     dicts with p, m (modified in place), lin, min, max and `mass` code.  sum_mode 1 =
-    exactly rounded sums, 0 = the reference's sequential order.  Returns (deficit, diag)."""
+    exactly rounded sums, 0 = the reference's sequential order.  flux: (mass_fo, mass_fi)
+    per tracer of the LAM Aranami flux (None: zero fluxes).  Returns (deficit, diag)."""
     minx, maxx, miny, maxy, nk = layout
     i0, in_, j0, jn, k0 = scope
     ntr = len(tracers)
@@ -337,7 +338,7 @@ def bermejo_conde(tracers, layout, scope, airm1, airm0, mask, deficit, pkps=None
     pl = np.asarray(pexp_list, np.float32)
     lib().ora_bermejo_conde(tdt, _p(dom), ntr, ptrs("p"), ptrs("m"), ptrs("lin"), ptrs("min"), ptrs("max"),
                             _p(legacy), _p(weight), _p(pexp_n), _p(de), _p(airm1), _p(airm0), _p(pk), _p(mask),
-                            _p(pl), _p(diag))
+                            _p(pl), _p(diag), None if flux is None else _p(np.ascontiguousarray(flux, np.float64)))
     return de, diag.reshape(ntr, 6)
 
 
@@ -346,3 +347,88 @@ def mass(x, air, mask, layout, scope, sum_mode=1):
     i0, in_, j0, jn, k0 = scope
     dom = np.array([minx, maxx, miny, maxy, nk, i0, in_, j0, jn, k0, 0, sum_mode], np.int32)
     return lib().ora_mass(1 if x.dtype == np.float32 else 0, _p(dom), _p(x), _p(air), _p(mask))
+
+
+# ------------------------------------------------------------------ LAM boundary flux
+def bc_lam_masks(layout, tile, core):
+    """adz_BC_LAM_setup.F90:18-92 for a tile whose four sides are LAM edges.
+
+    mask_o is 1 on the core (:50-62) and mask_i is 1 on the tile 1..ni x 1..nj x 1..nk
+    outside the core (:64-76).  Both are 0 in the halo (:80; one tile: nothing is exchanged
+    into it).  Returns two float32 arrays (nk, ny, nx) on `layout` = (minx, maxx, miny,
+    maxy, nk); core = (ic0, ic1, jc0, jc1, kc0)."""
+    minx, maxx, miny, maxy, nk = layout
+    ni, nj = tile
+    ic0, ic1, jc0, jc1, kc0 = core
+    shape = (nk, maxy - miny + 1, maxx - minx + 1)
+    one = np.zeros(shape, np.float32)
+    one[:, 1 - miny:nj + 1 - miny, 1 - minx:ni + 1 - minx] = 1.0           # fld_ONE(1:ni, 1:nj, 1:nk)
+    c = (slice(kc0 - 1, nk), slice(jc0 - miny, jc1 + 1 - miny), slice(ic0 - minx, ic1 + 1 - minx))
+    mo = np.zeros(shape, np.float32)
+    mo[c] = one[c]
+    mi = one.copy()
+    mi[c] = 0.0
+    return mo, mi
+
+
+def bc_lam_aranami(tracers, layout, tile, bbox, core, zt, pb, order=ORD_AVX2, zabs_fma=None, kernel=None):
+    """adz_BC_LAM_Aranami_hlt.F90:18-236 for a tile whose four sides are LAM edges.
+
+    tracers: flat arrays on `layout` (array start).  zt: Ztab over the layout (ni = row
+    pitch, no offsets); pb: Adz_pb (3, i0b:inb, j0b:jnb, 1:nk); bbox = (i0b, inb, j0b, jnb).
+    kernel: None (this oracle's Tricublin_zyx1_m_n) or (RefKernel, its ztab handle) to
+    interpolate with the reference's own compiled code.  Returns (fo, fi): lists of flat
+    arrays on `layout` holding the b-range values and zero elsewhere."""
+    minx, maxx, miny, maxy, nk = layout
+    ni, nj = tile
+    i0b, inb, j0b, jnb = bbox
+    ic0, ic1, jc0, jc1, kc0 = core
+    pil_w, pil_e, pil_s, pil_n = ic0 - 1, ni - ic1, jc0 - 1, nj - jc1
+    nx, ny = maxx - minx + 1, maxy - miny + 1
+    base = (1 - minx) + nx * (1 - miny)
+    mo, mi = bc_lam_masks(layout, tile, core)
+    tile_sl = (slice(None), slice(1 - miny, nj + 1 - miny), slice(1 - minx, ni + 1 - minx))  # [i0:in] x [j0:jn]
+    fields = []
+    for f in tracers:  # :58-82: adz_o / adz_i = mask * tracer, a product in the tracer's type
+        f3 = f.reshape(nk, ny, nx)
+        ao, ai = np.zeros_like(f3), np.zeros_like(f3)
+        ao[tile_sl] = mo[tile_sl].astype(f.dtype) * f3[tile_sl]
+        ai[tile_sl] = mi[tile_sl].astype(f.dtype) * f3[tile_sl]
+        fields += [ao.reshape(-1), ai.reshape(-1)]
+    # :110-155: one m_n interpolation of (adz_o, adz_i) per tracer at every Adz_pb target
+    if kernel is None:
+        d = zyx1_m_n(zt, fields, base, pb, order, zabs_fma)
+    else:
+        ref, lv = kernel
+        d = ref.zyx1_m_n(lv, fields, base, pb)
+    nbx, nby = inb - i0b + 1, jnb - j0b + 1
+    b_sl = (slice(None), slice(j0b - miny, jnb + 1 - miny), slice(i0b - minx, inb + 1 - minx))
+    J = lambda a, b: slice(a - miny, b + 1 - miny)  # noqa: E731  (j = a..b)
+    I = lambda a, b: slice(a - minx, b + 1 - minx)  # noqa: E731
+    fo, fi = [], []
+    for n, f in enumerate(tracers):
+        o = np.zeros((nk, ny, nx), f.dtype)
+        i_ = np.zeros((nk, ny, nx), f.dtype)
+        o[b_sl] = d[2 * n].reshape(nk, nby, nbx)
+        i_[b_sl] = d[2 * n + 1].reshape(nk, nby, nbx)
+        o[kc0 - 1:, J(1 + pil_s, nj - pil_n), I(1 + pil_w, ni - pil_e)] = 0     # :161-169
+        i_[kc0 - 1:, J(nj - pil_n + 1, nj), I(1, ni)] = 0                        # :171-181 (l_north)
+        i_[kc0 - 1:, J(1, nj), I(ni - pil_e + 1, ni)] = 0                        # :183-193 (l_east)
+        i_[kc0 - 1:, J(1, pil_s), I(1, ni)] = 0                                  # :195-205 (l_south)
+        i_[kc0 - 1:, J(1, nj), I(1, pil_w)] = 0                                  # :207-217 (l_west)
+        i_[:kc0 - 1, J(j0b, jnb), I(i0b, inb)] = 0                               # :219-227
+        fo.append(o.reshape(-1))
+        fi.append(i_.reshape(-1))
+    return fo, fi
+
+
+def flux_masses(fo, fi, airm0, mask, layout, bbox, sum_mode=1):
+    """mass_tr_PM.F90:101-127: masses of FLUX_out and FLUX_in over Adz_i0b:Adz_inb x
+    Adz_j0b:Adz_jnb x 1:nk with the air mass at time M.  Returns [mass_fo, mass_fi] per
+    tracer, flattened."""
+    i0b, inb, j0b, jnb = bbox
+    scope = (i0b, inb, j0b, jnb, 1)
+    out = []
+    for o, i_ in zip(fo, fi):
+        out += [mass(o, airm0, mask, layout, scope, sum_mode), mass(i_, airm0, mask, layout, scope, sum_mode)]
+    return np.array(out)

@@ -577,11 +577,13 @@ inline double mass_of(const BCDomain& D, const T* x, const A* air) {
   return D.sum_mode == 0 ? acc : fsum(terms);
 }
 
-// Bermejo_Conde.F90:18-320 with Adz_BC_LAM_flux fluxes zero (mass_fo = mass_fi = 0),
-// reset_epsilon_L = .true.  w is computed into `w` (caller storage, same layout as m).
+// Bermejo_Conde.F90:18-320 with reset_epsilon_L = .true.  flux: the masses (mass_fo,
+// mass_fi) per tracer of the LAM Aranami flux (mass_tr_PM.F90:101-127), or null for zero
+// fluxes.  w is computed into `w` (caller storage, same layout as m).
 template <class T>
 inline void bermejo_conde(const BCDomain& D, std::vector<BCTracer<T>>& tr, const T* airm1, const T* airm0,
-                          const T* pkps, std::vector<std::vector<T>>& w, double* diag /* 6 per tracer or null */) {
+                          const T* pkps, std::vector<std::vector<T>>& w, double* diag /* 6 per tracer or null */,
+                          const double* flux = nullptr) {
   const int ntr = (int)tr.size();
   std::vector<double> mass_p(ntr), mass_m(ntr), mass_tot_p(ntr), mass_deficit(ntr), mass_wei(ntr);
   std::vector<int> cycle_tr(ntr, 0);
@@ -595,7 +597,7 @@ inline void bermejo_conde(const BCDomain& D, std::vector<BCTracer<T>>& tr, const
   }
   for (int n = 0; n < ntr; n++) {
     double p_exp = tr[n].legacy ? 1.0 : (double)D.pexp_list[tr[n].pexp_n - 1];
-    double mass_bflux = 0.0 - 0.0;
+    double mass_bflux = flux ? flux[2 * n + 1] - flux[2 * n] : 0.0 - 0.0;  // :92
     mass_p[n] = mass_p[n] - tr[n].mass_deficit;
     mass_tot_p[n] = mass_p[n] + mass_bflux;
     double mass_tot_m = mass_m[n];
