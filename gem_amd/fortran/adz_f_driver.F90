@@ -2,7 +2,7 @@
 ! through the drop-in routines of module adz_gpu, the way gemdyn's tstpdyn / adz_main /
 ! adz_tracers sequence calls them:
 !   adz_set -> adz_inittraj -> adz_traject(dt) -> adz_tricub_hlt(tracers at Adz_pt, with
-!   Adz_post) -> adz_post_tr(1) -> adz_post_tr(0) (Bermejo-Conde)
+!   Adz_post, with the LAM boundary flux) -> adz_post_tr(1) -> adz_post_tr(0) (Bermejo-Conde)
 !
 ! Usage: adz_f_driver <input file> <output file>
 !        adz_f_driver sizes      (prints the byte sizes of the bind(C) types)
@@ -137,14 +137,17 @@ program adz_f_driver
 contains
 
    subroutine print_sizes()
-      use adz_hip, only: adz_field, adz_item, adz_tune, adz_bc_tracer, adz_bc_domain
+      use adz_hip, only: adz_field, adz_item, adz_tune, adz_bc_tracer, adz_bc_domain, adz_flux_tracer, &
+                         adz_flux_domain
       type(adz_field) :: f
       type(adz_item) :: it
       type(adz_tune) :: t
       type(adz_bc_tracer) :: b
       type(adz_bc_domain) :: d
-      write (*, '(a,6(1x,i0))') 'sizes', c_sizeof(g), c_sizeof(f), c_sizeof(it), c_sizeof(t), c_sizeof(b), &
-         c_sizeof(d)
+      type(adz_flux_tracer) :: ft
+      type(adz_flux_domain) :: fd
+      write (*, '(a,8(1x,i0))') 'sizes', c_sizeof(g), c_sizeof(f), c_sizeof(it), c_sizeof(t), c_sizeof(b), &
+         c_sizeof(d), c_sizeof(ft), c_sizeof(fd)
    end subroutine print_sizes
 
    function c_ptr_add(p, bytes) result(q)

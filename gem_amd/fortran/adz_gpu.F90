@@ -9,6 +9,8 @@
 !                                    F_i0, F_in, F_j0, F_jn, F_k0, F_ext_L, F_post, F_QV_L)
 !   adz_post_tr.F90:18-276      -> adz_post_tr(F_wp)            (same signature)
 !   Bermejo_Conde.F90:18-320    -> Bermejo_Conde(F_bc, F_ntr_bc, F_i0, F_in, F_j0, F_jn, F_k0)
+!   adz_BC_LAM_Aranami_hlt.F90:18-236 -> adz_BC_LAM_Aranami_hlt(F_stk, F_post, F_nptr), called by
+!                                    adz_tricub_hlt with F_post on a LAM with Adz_BC_LAM_flux = 1
 !
 ! The module variables stand in for what the reference keeps in adz_mem / adz_options /
 ! dyn_fisl_options / cstv / tr3d (same names).  Array-valued members that the reference
@@ -25,13 +27,17 @@
 !     part of this library: they stop with a message.
 !   * Bermejo_Conde synchronises the device to return each tracer's mass_deficit in
 !     F_bc(n)%mass_deficit, as the reference's host scalar.
+!   * The LAM boundary flux keeps only what Bermejo_Conde reads of it: the masses of
+!     FLUX_out / FLUX_in per tracer (mass_tr_PM.F90:101-127, with the air mass at time M
+!     and geomh_mask_8 of the module state when adz_tricub_hlt runs), not the Adz_flux
+!     arrays themselves.
 module adz_gpu
    use, intrinsic :: iso_c_binding
    use, intrinsic :: iso_fortran_env, only: REAL64, error_unit
    use adz_hip, c_adz_set => adz_set, c_adz_inittraj => adz_inittraj, c_adz_traject => adz_traject, &
-                c_adz_tricub_hlt => adz_tricub_hlt
+                c_adz_tricub_hlt => adz_tricub_hlt, c_adz_bc_lam_flux => adz_bc_lam_flux
    implicit none
-   private :: c_adz_set, c_adz_inittraj, c_adz_traject, c_adz_tricub_hlt
+   private :: c_adz_set, c_adz_inittraj, c_adz_traject, c_adz_tricub_hlt, c_adz_bc_lam_flux
 
    ! ------------------------------------------------------------ types (adz_mem.F90:66-125, tr3d_mod.F90:63-70)
    type :: Adz_pntr_stack
@@ -91,6 +97,8 @@ module adz_gpu
    type(post), allocatable :: Adz_post(:)
    type(bc), allocatable :: Adz_bc(:)
    integer, private, save :: n_bc_TOTAL = 0
+   ! masses (mass_fo, mass_fi) of the Aranami flux per tracer of the last adz_tricub_hlt
+   real(kind=REAL64), allocatable :: Adz_flux_mass(:, :)
 
 contains
 
@@ -176,6 +184,10 @@ contains
          end if
       end if
       if ((F_in - F_i0 + 1) <= 0 .or. (F_jn - F_j0 + 1) <= 0) return
+      ! adz_interp_hlt.F90:91-96: the LAM boundary flux of the fixer's tracers
+      if (.not. ext_L .and. present(F_post)) then
+         if (Adz_BC_LAM_flux == 1) call adz_BC_LAM_Aranami_hlt(F_stk, F_post, F_nptr)
+      end if
       if (F_num /= (F_in - F_i0 + 1) * (F_jn - F_j0 + 1) * (l_nk - F_k0 + 1)) then
          write (error_unit, *) 'adz_tricub_hlt: F_num does not match the target box'
          error stop 1
@@ -215,6 +227,52 @@ contains
       end do
       deallocate (fl)
    end subroutine adz_tricub_hlt
+
+   ! adz_BC_LAM_Aranami_hlt.F90:18-236 for the tracers of F_post that use the fixer (mass 1
+   ! or 111..139): FLUX_out / FLUX_in at Adz_pb and their masses into Adz_flux_mass(:, n).
+   subroutine adz_BC_LAM_Aranami_hlt(F_stk, F_post, F_nptr)
+      integer, intent(in) :: F_nptr
+      type(Adz_pntr_stack), dimension(F_nptr), intent(in) :: F_stk
+      type(meta_tracers), dimension(F_nptr), intent(in) :: F_post
+      type(adz_flux_tracer), allocatable :: tr(:)
+      type(adz_flux_domain) :: dom
+      integer, allocatable :: tr_num(:)
+      real(kind=REAL64), allocatable, target :: hm(:)
+      type(c_ptr) :: h, d_mass
+      integer(c_size_t) :: bytes
+      integer :: n, nf
+      if (allocated(Adz_flux_mass)) deallocate (Adz_flux_mass)
+      allocate (Adz_flux_mass(2, F_nptr), tr(F_nptr), tr_num(F_nptr))
+      Adz_flux_mass = 0.d0
+      nf = 0
+      do n = 1, F_nptr
+         if (.not. (F_post(n)%mass == 1 .or. (F_post(n)%mass >= 111 .and. F_post(n)%mass <= 139))) cycle
+         nf = nf + 1
+         tr_num(nf) = n
+         tr(nf)%src = F_stk(n)%src
+      end do
+      if (nf == 0) return
+      if (.not. (c_associated(airm0) .and. c_associated(geomh_mask_8))) then
+         write (error_unit, *) 'adz_BC_LAM_Aranami_hlt: airm0 and geomh_mask_8 must be set before the tracers'
+         error stop 1
+      end if
+      dom = adz_flux_domain(l_minx, l_maxx, l_miny, l_maxy, l_nk, l_ni, l_nj, Adz_i0b, Adz_inb, Adz_j0b, Adz_jnb, &
+                            1 + pil_w, l_ni - pil_e, 1 + pil_s, l_nj - pil_n, Adz_k0t)
+      call adz_check(adz_flux_create(tr, int(nf, c_int), dom, Adz_cpntr_t, int(Adz_field_dtype, c_int), &
+                                     int(Adz_pos_dtype, c_int), h), 'adz_flux_create')
+      bytes = int(16, c_size_t) * int(nf, c_size_t)
+      if (hipMalloc(d_mass, bytes) /= 0) call adz_check(ADZ_ENOMEM, 'adz_BC_LAM_Aranami_hlt: hipMalloc')
+      call adz_check(c_adz_bc_lam_flux(h, Adz_pb, airm0, geomh_mask_8, d_mass, Adz_stream), 'adz_bc_lam_flux')
+      allocate (hm(2 * nf))
+      if (hipMemcpy(c_loc(hm), d_mass, bytes, hipMemcpyDeviceToHost) /= 0) &
+         call adz_check(ADZ_EHIP, 'adz_BC_LAM_Aranami_hlt: hipMemcpy')
+      do n = 1, nf
+         Adz_flux_mass(:, tr_num(n)) = hm(2 * n - 1:2 * n)
+      end do
+      if (hipFree(d_mass) /= 0) call adz_check(ADZ_EHIP, 'adz_BC_LAM_Aranami_hlt: hipFree')
+      call adz_check(adz_flux_free(h), 'adz_flux_free')
+      deallocate (tr, tr_num, hm)
+   end subroutine adz_BC_LAM_Aranami_hlt
 
    ! F_wp = 1: register the Tr_3CWP tracers that use the fixer (their clipping happened in
    ! adz_tricub_hlt); F_wp = 0: core limits, settings from each tracer's mass code,
@@ -274,12 +332,16 @@ contains
       type(bc), intent(inout), dimension(F_ntr_bc) :: F_bc
       type(adz_bc_tracer), allocatable :: tr(:)
       type(adz_bc_domain) :: dom
-      type(c_ptr) :: h, d_def
-      real(kind=REAL64), allocatable, target :: def(:)
+      type(c_ptr) :: h, d_def, d_flux
+      real(kind=REAL64), allocatable, target :: def(:), flux(:, :)
       integer(c_size_t) :: bytes
       integer :: n
+      logical :: lam_flux_L
       if (F_ntr_bc <= 0) return
-      allocate (tr(F_ntr_bc), def(F_ntr_bc))
+      ! Bermejo_Conde.F90:70-72,92: the Aranami flux masses of each tracer (zero otherwise)
+      lam_flux_L = Adz_BC_LAM_flux == 1 .and. allocated(Adz_flux_mass)
+      allocate (tr(F_ntr_bc), def(F_ntr_bc), flux(2, F_ntr_bc))
+      flux = 0.d0
       do n = 1, F_ntr_bc
          tr(n)%p = F_bc(n)%p
          tr(n)%m = F_bc(n)%m
@@ -290,6 +352,9 @@ contains
          tr(n)%weight = F_bc(n)%weight
          tr(n)%pexp_n = F_bc(n)%pexp_n
          def(n) = F_bc(n)%mass_deficit
+         if (lam_flux_L) then
+            if (F_bc(n)%n >= 1 .and. F_bc(n)%n <= size(Adz_flux_mass, 2)) flux(:, n) = Adz_flux_mass(:, F_bc(n)%n)
+         end if
       end do
       dom = adz_bc_domain(l_minx, l_maxx, l_miny, l_maxy, l_nk, F_i0, F_in, F_j0, F_jn, F_k0, &
                           merge(1, 0, Adz_BC_min_max_L), Adz_BC_pexp_list)
@@ -298,16 +363,27 @@ contains
       if (hipMemcpy(d_def, c_loc(def), bytes, hipMemcpyHostToDevice) /= 0) &
          call adz_check(ADZ_EHIP, 'Bermejo_Conde: hipMemcpy')
       call adz_check(adz_bc_create(tr, int(F_ntr_bc, c_int), dom, int(Adz_field_dtype, c_int), h), 'adz_bc_create')
-      call adz_check(adz_bermejo_conde(h, airm1, airm0, pkps, geomh_mask_8, d_def, c_null_ptr, Adz_stream), &
-                     'adz_bermejo_conde')
+      if (lam_flux_L) then
+         if (hipMalloc(d_flux, 2 * bytes) /= 0) call adz_check(ADZ_ENOMEM, 'Bermejo_Conde: hipMalloc')
+         if (hipMemcpy(d_flux, c_loc(flux), 2 * bytes, hipMemcpyHostToDevice) /= 0) &
+            call adz_check(ADZ_EHIP, 'Bermejo_Conde: hipMemcpy')
+         call adz_check(adz_bermejo_conde_lam(h, airm1, airm0, pkps, geomh_mask_8, d_flux, d_def, c_null_ptr, &
+                                              Adz_stream), 'adz_bermejo_conde_lam')
+      else
+         call adz_check(adz_bermejo_conde(h, airm1, airm0, pkps, geomh_mask_8, d_def, c_null_ptr, Adz_stream), &
+                        'adz_bermejo_conde')
+      end if
       if (hipMemcpy(c_loc(def), d_def, bytes, hipMemcpyDeviceToHost) /= 0) &
          call adz_check(ADZ_EHIP, 'Bermejo_Conde: hipMemcpy')
       call adz_check(adz_bc_free(h), 'adz_bc_free')
       if (hipFree(d_def) /= 0) call adz_check(ADZ_EHIP, 'Bermejo_Conde: hipFree')
+      if (lam_flux_L) then
+         if (hipFree(d_flux) /= 0) call adz_check(ADZ_EHIP, 'Bermejo_Conde: hipFree')
+      end if
       do n = 1, F_ntr_bc
          F_bc(n)%mass_deficit = def(n)
       end do
-      deallocate (tr, def)
+      deallocate (tr, def, flux)
    end subroutine Bermejo_Conde
 
 end module adz_gpu

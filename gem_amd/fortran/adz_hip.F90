@@ -60,6 +60,17 @@ module adz_hip
       real(c_float)  :: pexp_list(9)
    end type adz_bc_domain
 
+   type, bind(C) :: adz_flux_tracer
+      type(c_ptr) :: src = c_null_ptr, fo = c_null_ptr, fi = c_null_ptr
+   end type adz_flux_tracer
+
+   type, bind(C) :: adz_flux_domain
+      integer(c_int) :: minx, maxx, miny, maxy, nk
+      integer(c_int) :: ni, nj
+      integer(c_int) :: i0b, inb, j0b, jnb
+      integer(c_int) :: ic0, ic1, jc0, jc1, kc0
+   end type adz_flux_domain
+
    interface
       function adz_last_error() bind(C, name="adz_last_error") result(msg)
          import :: c_ptr
@@ -235,9 +246,42 @@ module adz_hip
          type(c_ptr), value :: bc, airm1, airm0, pkps, mask, deficit, diag, stream
          integer(c_int) :: rc
       end function
+      function adz_bermejo_conde_lam(bc, airm1, airm0, pkps, mask, flux_mass, deficit, diag, stream) &
+            bind(C, name="adz_bermejo_conde_lam") result(rc)
+         import :: c_int, c_ptr
+         type(c_ptr), value :: bc, airm1, airm0, pkps, mask, flux_mass, deficit, diag, stream
+         integer(c_int) :: rc
+      end function
       function adz_bc_free(bc) bind(C, name="adz_bc_free") result(rc)
          import :: c_int, c_ptr
          type(c_ptr), value :: bc
+         integer(c_int) :: rc
+      end function
+
+      ! ---------------------------------------------------------------- level 4b
+      function adz_flux_create(tr, ntr, dom, geom, tdtype, pdtype, fx) bind(C, name="adz_flux_create") result(rc)
+         import :: c_int, c_ptr, adz_flux_tracer, adz_flux_domain
+         type(adz_flux_tracer), intent(in) :: tr(*)
+         integer(c_int), value :: ntr, tdtype, pdtype
+         type(adz_flux_domain), intent(in) :: dom
+         type(c_ptr), value :: geom
+         type(c_ptr), intent(out) :: fx
+         integer(c_int) :: rc
+      end function
+      function adz_bc_lam_flux(fx, xyz, airm0, mask, mass, stream) bind(C, name="adz_bc_lam_flux") result(rc)
+         import :: c_int, c_ptr
+         type(c_ptr), value :: fx, xyz, airm0, mask, mass, stream
+         integer(c_int) :: rc
+      end function
+      function adz_flux_active(fx, count) bind(C, name="adz_flux_active") result(rc)
+         import :: c_int, c_ptr
+         type(c_ptr), value :: fx
+         integer(c_int), intent(out) :: count
+         integer(c_int) :: rc
+      end function
+      function adz_flux_free(fx) bind(C, name="adz_flux_free") result(rc)
+         import :: c_int, c_ptr
+         type(c_ptr), value :: fx
          integer(c_int) :: rc
       end function
    end interface

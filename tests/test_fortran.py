@@ -20,4 +20,4 @@ def test_bind_c_type_sizes_match_c_abi():
     line = [ln for ln in out.stdout.splitlines() if ln.startswith("sizes")][0]
     sizes = [int(x) for x in line.split()[1:]]
     assert sizes == [C.sizeof(s) for s in (A.AdzGrid, A.AdzField, A.AdzItem, A.AdzTune, A.AdzBcTracer,
-                                           A.AdzBcDomain)]
+                                           A.AdzBcDomain, A.AdzFluxTracer, A.AdzFluxDomain)]

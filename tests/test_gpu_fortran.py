@@ -3,7 +3,8 @@ tracer step through the drop-in routines with the reference's signatures -- adz_
 adz_inittraj, adz_traject(F_dt_8), adz_tricub_hlt(F_stk, ..., F_post), adz_post_tr(1),
 adz_post_tr(0) -> Bermejo_Conde -- and its results are bit-identical to the same step
 called through the C ABI from Python: departure positions, interpolated and clipped
-tracers, Adz_post lin/min/max, fixed tracers and the carried mass deficits."""
+tracers, Adz_post lin/min/max, fixed tracers and the carried mass deficits (the fixer with
+the LAM boundary flux, Adz_BC_LAM_flux = 1)."""
 import os
 import subprocess
 
@@ -86,8 +87,13 @@ def test_fortran_step_matches_c_abi(cuda, tmp_path):
     scope = (1 + g["pil_w"], cfg.ni - g["pil_e"], 1 + g["pil_s"], cfg.nj - g["pil_n"], i["k0t"])
     bc = A.BermejoConde(bcs, dl + (nk,), scope, off, min_max=True)
     deficit = torch.zeros(len(bcn), dtype=torch.float64, device="cuda")
-    bc(airm1.reshape(-1).cuda(), airm0.reshape(-1).cuda(), mask.reshape(-1).double().cuda(), deficit,
-       pkps=pkps.reshape(-1).cuda())
+    d_air0, d_mask = airm0.reshape(-1).cuda(), mask.reshape(-1).double().cuda()
+    # the LAM boundary flux (Adz_BC_LAM_flux = 1, the driver's grid) of the fixer's tracers
+    fx = A.BcLamFlux([dict(src=ent[n]["src"]) for n in bcn], dl + (nk,), (cfg.ni, cfg.nj),
+                     (i["i0b"], i["inb"], i["j0b"], i["jnb"]), scope, off, ctx.ztab(True), pos_dtype=L.ADZ_F32)
+    fmass = torch.zeros(2 * len(bcn), dtype=torch.float64, device="cuda")
+    fx(ctx.array(A.ADZ_A_PB), d_air0, d_mask, fmass)
+    bc(airm1.reshape(-1).cuda(), d_air0, d_mask, deficit, pkps=pkps.reshape(-1).cuda(), flux=fmass)
     torch.cuda.synchronize()
 
     npt = 3 * (box[1] - box[0] + 1) * (box[3] - box[2] + 1) * (nk - box[4] + 1)
@@ -114,5 +120,7 @@ def test_fortran_step_matches_c_abi(cuda, tmp_path):
     exp[bcn] = deficit.cpu().numpy()
     assert np.array_equal(fdef.view(np.uint64), exp.view(np.uint64))
     assert np.any(exp != 0.0)
+    assert np.all(fmass.cpu().numpy() != 0.0)
+    fx.free()
     bc.free()
     ctx.free()

@@ -15,8 +15,14 @@ target mass:
 with the carried deficit eps(n-1) = M_final(n-1) - M_target(n-1)
 (Bermejo_Conde.F90:100-105,292-308).
 
-The test checks, over 50 steps:
-- M_target stays within 1e-13 of its first value;
+With the LAM boundary flux (Adz_BC_LAM_flux = 1) the budget moves each step by the net
+flux into the core, mass_fi - mass_fo (Bermejo_Conde.F90:92-105):
+
+    M_target(n) = M_target(n-1) + mass_fi(n) - mass_fo(n).
+
+The test checks, over 50 steps, without and with the flux:
+- M_target stays within 1e-13 of its first value (with the flux: of the previous target
+  plus the net flux);
 - the fixer reaches the target to FP32 storage precision;
 - the carried deficit identity holds exactly;
 - the tracers stay finite.
@@ -31,7 +37,8 @@ MH = 2
 NSTEPS = 50
 
 
-def test_fifty_steps_conserve_mass(cuda):
+@pytest.mark.parametrize("lam_flux", [False, True])
+def test_fifty_steps_conserve_mass(cuda, lam_flux):
     import torch
 
     from gem_amd import _lib as L
@@ -64,22 +71,39 @@ def test_fifty_steps_conserve_mass(cuda):
         stacks.append(st)
         bcs.append(A.BermejoConde([dict(p=e["src"], m=e["dst"], lin=e["lin"], min=e["min"], max=e["max"], mass=1)
                                    for e in st], dl + (cfg.nk,), scope, off, min_max=True))
+    fxs = [A.BcLamFlux([dict(src=e["src"]) for e in st], dl + (cfg.nk,), (cfg.ni, cfg.nj),
+                       (i["i0b"], i["inb"], i["j0b"], i["jnb"]), scope, off, ctx.ztab(True), pos_dtype=L.ADZ_F64)
+           for st in stacks] if lam_flux else None
+    fmass = torch.zeros(2 * ntr, dtype=torch.float64, device="cuda")
     deficit = torch.zeros(ntr, dtype=torch.float64, device="cuda")
     diag = torch.zeros(6 * ntr, dtype=torch.float64, device="cuda")
-    hist = []
+    hist, fhist = [], []
     ctx.adz_inittraj()
     for n in range(NSTEPS):
         par = n % 2
         ctx.adz_traject(cfg.dt, w["uu_ext"], w["vv_ext"], w["ww_ext"], w["ut0"], w["vt0"], w["zdt0"])
         A.adz_tricub_hlt(stacks[par], ctx.array(A.ADZ_A_PT), ctx.ztab(True), *box[:5], box[5], dl, ext=False,
                          post=A.ADZ_POST_STORE, math=L.ADZ_MATH_F32, src_off=off)
-        bcs[par](air[par], air[1 - par], mask, deficit, pkps=pkps, diag=diag)
+        if lam_flux:
+            fxs[par](ctx.array(A.ADZ_A_PB), air[1 - par], mask, fmass)
+            bcs[par](air[par], air[1 - par], mask, deficit, pkps=pkps, diag=diag, flux=fmass)
+            fhist.append(fmass.cpu().numpy().reshape(ntr, 2).copy())
+        else:
+            bcs[par](air[par], air[1 - par], mask, deficit, pkps=pkps, diag=diag)
         hist.append(diag.cpu().numpy().reshape(ntr, 6).copy())
     torch.cuda.synchronize()
     h = np.stack(hist)                       # [step, tracer, (target, before, W, final, skipped, deficit)]
     target, final, skipped, eps = h[..., 0], h[..., 3], h[..., 4], h[..., 5]
     assert not skipped.any()
-    drift = np.abs(target - target[0]).max(axis=0) / np.abs(target[0])
+    if lam_flux:
+        f = np.stack(fhist)                  # [step, tracer, (mass_fo, mass_fi)]
+        net = f[..., 1] - f[..., 0]
+        drift = np.abs(target[1:] - (target[:-1] + net[1:])).max(axis=0) / np.abs(target[0])
+        print(f"[mass] {NSTEPS} steps with the LAM flux: net flux per step up to "
+              f"{np.abs(net).max(axis=0) / np.abs(target[0])} of the budget", flush=True)
+        assert np.all(np.abs(net[1:]) > 0.0)
+    else:
+        drift = np.abs(target - target[0]).max(axis=0) / np.abs(target[0])
     print(f"[mass] {NSTEPS} steps: max relative drift of the conserved budget per tracer {drift}", flush=True)
     print(f"[mass] max |final - target| / target: {np.abs(final - target).max() / np.abs(target).min():.3e}",
           flush=True)
@@ -88,6 +112,6 @@ def test_fifty_steps_conserve_mass(cuda):
     assert np.all(np.abs(final - target) <= 1e-6 * np.abs(target))
     for t in range(ntr):
         assert bool(torch.isfinite(bufs[t][NSTEPS % 2]).all())
-    for b in bcs:
+    for b in bcs + (fxs or []):
         b.free()
     ctx.free()
