@@ -23,8 +23,9 @@
 !   * With F_post present, tracers whose meta has mono == 1 are clipped to [min, max]
 !     inside the interpolation launch (the same operation adz_post_tr.F90:81-94 applies to
 !     the same box afterwards), so adz_post_tr(1) only registers the fixer's tracers.
-!   * ILMC (mono = 2), quintic vertical (F_QV_L) and the BICHQV group (F_wp = 2) are not
-!     part of this library: they stop with a message.
+!   * ILMC (mono = 2), quintic vertical (F_QV_L), the BICHQV group (F_wp = 2) and the ZLF
+!     boundary flux (Adz_BC_LAM_flux = 2) are not part of this library: they stop with a
+!     message.
 !   * Bermejo_Conde synchronises the device to return each tracer's mass_deficit in
 !     F_bc(n)%mass_deficit, as the reference's host scalar.
 !   * The LAM boundary flux keeps only what Bermejo_Conde reads of it: the masses of
@@ -300,10 +301,11 @@ contains
          return
       end if
       if (n_bc_TOTAL == 0) return
-      i0 = 1 + pil_w; in = l_ni - pil_e; j0 = 1 + pil_s; jn = l_nj - pil_n; k0 = Adz_k0t
-      if (Adz_BC_LAM_flux == 2) then
-         i0 = Adz_i0b; in = Adz_inb; j0 = Adz_j0b; jn = Adz_jnb; k0 = 1
+      if (Adz_BC_LAM_flux == 2) then  ! adz_post_tr.F90:147-151 and adz_BC_LAM_zlf_*
+         write (error_unit, *) 'adz_post_tr: the ZLF boundary flux (Adz_BC_LAM_flux = 2) is not part of this library'
+         error stop 1
       end if
+      i0 = 1 + pil_w; in = l_ni - pil_e; j0 = 1 + pil_s; jn = l_nj - pil_n; k0 = Adz_k0t
       do num = 1, n_bc_TOTAL
          n = Adz_bc(num)%n
          Adz_bc(num)%p = Adz_stack(n)%src
