@@ -54,6 +54,8 @@ def parse():
     p.add_argument("--traj-storage", choices=("f32", "f64"), default="f32",
                    help="storage of the trajectory state: f32 = the reference's real arrays (FP64 iteration "
                         "arithmetic and FP64 position history), f64 = FP64 throughout (timed beside it either way)")
+    p.add_argument("--tiles", default=os.path.join(ROOT, "gem_amd", "tuned_tiles.json"),
+                   help="JSON file of tile parameters per workload (the autotuner's committed result)")
     p.add_argument("--retune", action="store_true",
                    help="run adz_plan_autotune instead of using the tiles it found before (gem_amd/tuned_tiles.json)")
     return p.parse_args()
@@ -155,7 +157,7 @@ def main():
               item("t", [dict(src=src[2], dst=dst[2])], True),
               item("t", [dict(src=src[3], dst=dst[3])], False)]
     t_tune = time.time()
-    cached = None if args.retune else tuned_tiles(args.config)  # the autotuner's earlier result
+    cached = None if args.retune else tuned_tiles(args.config, args.tiles)  # the autotuner's earlier result
     tuneA = cached["uvtq"] if cached else A.Plan.autotune(itemsA, L.ADZ_F32, math=L.ADZ_MATH_F32, post=0, reps=2)[0]
     planA = A.Plan(itemsA, L.ADZ_F32, math=L.ADZ_MATH_F32, post=0, tune=tuneA)
     # plans B: tracer groups with lin/min/max for the fixer, and the fixer itself
@@ -433,7 +435,7 @@ def run_c4(args, np, torch, L, A, S, dev):
     log(f"c4: {len(state)} subdomains, {pts / 1e6:.1f} M target points, {pf / 1e9:.2f} G point-fields per step, "
         f"setup {time.time() - t_setup:.1f}s")
     t0 = time.time()
-    cached = None if args.retune else tuned_tiles("c4")
+    cached = None if args.retune else tuned_tiles("c4", args.tiles)
     if cached:
         tune = cached["plan"]
         log(f"tiles from gem_amd/tuned_tiles.json: {tune}")
@@ -543,11 +545,11 @@ def run_c4(args, np, torch, L, A, S, dev):
     default_plan.free()
 
 
-def tuned_tiles(config):
+def tuned_tiles(config, path):
     """Tile parameters adz_plan_autotune found for this workload (committed beside the
     package), or None."""
     try:
-        with open(os.path.join(ROOT, "gem_amd", "tuned_tiles.json")) as f:
+        with open(path) as f:
             return json.load(f).get(config)
     except (OSError, ValueError):
         return None

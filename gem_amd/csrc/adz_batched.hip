@@ -224,6 +224,42 @@ __device__ __forceinline__ void per_lane(const DevItem& it, const Fields& F, int
   }
 }
 
+// Tile order within an item (ptile -> tile column ib, tile row jb, level kk):
+//   level_inner 0: i fastest, then j, then k;
+//   level_inner 1: the levels of one tile column first, then i, then j;
+//   level_inner S >= 2: super-columns of S tile rows (4*S target rows) x every level, each
+//     walked level by level (the S tile rows of a level together), super-columns along i,
+//     then j.  Consecutive tiles then share source planes (level to level) and source rows
+//     (within the super-column), which keeps the blocks resident on one XCD inside a
+//     working set its L2 can hold.  Tiles past the last row (njb not a multiple of S) hold
+//     no valid target.
+__device__ __forceinline__ void tile_coords(const DevItem& it, long ptile, int level_inner, int* ib, int* jb,
+                                            int* kk) {
+  const long nib = (it.nx + kTI - 1) / kTI, njb = (it.ny + kTJ - 1) / kTJ;
+  if (level_inner >= 2) {
+    const long S = level_inner, per = (long)it.nz * S;
+    const long sc = ptile / per, r = ptile % per;
+    *kk = (int)(r / S);
+    *ib = (int)(sc % nib);
+    *jb = (int)((sc / nib) * S + r % S);
+  } else if (level_inner) {
+    *kk = (int)(ptile % it.nz);
+    *ib = (int)((ptile / it.nz) % nib);
+    *jb = (int)(ptile / (it.nz * nib));
+  } else {
+    *ib = (int)(ptile % nib);
+    *jb = (int)((ptile / nib) % njb);
+    *kk = (int)(ptile / (nib * njb));
+  }
+}
+
+// tiles of an item in the order above
+inline long item_ptiles(int nx, int ny, int nz, int level_inner) {
+  const long nib = (nx + kTI - 1) / kTI, njb = (ny + kTJ - 1) / kTJ;
+  if (level_inner >= 2) return nib * ((njb + level_inner - 1) / level_inner) * level_inner * nz;
+  return nib * njb * nz;
+}
+
 // One tile: TI x TJ targets (i fastest) of one level of item `it`, fields [f0, f1).
 // The union of the tile's stencils (its "box") is staged in LDS once per field with
 // coalesced loads, then every lane evaluates its stencil from LDS.  A box larger than
@@ -233,17 +269,8 @@ template <class TF, class TP, int MATH, int POST, class Fields>
 __device__ __forceinline__ void run_tile(const DevItem& it, const Fields& F, long ptile, int f0, int f1, TF* boxA,
                                          TF* boxB, int* red, int* rowoff, int stage, int level_inner) {
   const int lane = threadIdx.x & (kTI - 1), wv = threadIdx.x / kTI;
-  const long nib = (it.nx + kTI - 1) / kTI, njb = (it.ny + kTJ - 1) / kTJ;
   int ib, jb, kk;
-  if (level_inner) {  // consecutive tiles: the levels of one column (vertical reuse in L2)
-    kk = (int)(ptile % it.nz);
-    ib = (int)((ptile / it.nz) % nib);
-    jb = (int)(ptile / (it.nz * nib));
-  } else {            // consecutive tiles: along i, then j, then k
-    ib = (int)(ptile % nib);
-    jb = (int)((ptile / nib) % njb);
-    kk = (int)(ptile / (nib * njb));
-  }
+  tile_coords(it, ptile, level_inner, &ib, &jb, &kk);
   const int ii = ib * kTI + lane, jj = jb * kTJ + wv;
   const bool valid = ii < it.nx && jj < it.ny;
   const int i = it.i0 + ii, j = it.j0 + jj, k = it.k0 + kk;
@@ -404,13 +431,8 @@ __device__ __forceinline__ void pc_tile(const DevItem& it, const ItemFields& F, 
                                         lds_flag* abort_flag, int level_inner) {
   const int wave = threadIdx.x / 64, lane = threadIdx.x & 63;
   const bool consumer = wave > 0;
-  const long nib = (it.nx + kTI - 1) / kTI, njb = (it.ny + kTJ - 1) / kTJ;
   int ib, jb, kk;
-  if (level_inner) {
-    kk = (int)(ptile % it.nz); ib = (int)((ptile / it.nz) % nib); jb = (int)(ptile / (it.nz * nib));
-  } else {
-    ib = (int)(ptile % nib); jb = (int)((ptile / nib) % njb); kk = (int)(ptile / (nib * njb));
-  }
+  tile_coords(it, ptile, level_inner, &ib, &jb, &kk);
   const int ii = ib * kTI + lane, jj = jb * kTJ + (wave - 1);
   const bool valid = consumer && ii < it.nx && jj < it.ny;
   const int i = it.i0 + ii, j = it.j0 + jj, k = it.k0 + kk;
@@ -799,6 +821,7 @@ int adz_plan_create(const adz_item* items, int nitems, int fdtype, int math, int
       ptrs[4 * nptr_total + pofs + f] = a.fld[f].mx;
     }
     dv[q].ngroups = (a.nptr + T.fields_per_pass - 1) / T.fields_per_pass;
+    dv[q].ptiles = item_ptiles(dv[q].nx, dv[q].ny, dv[q].nz, T.level_inner);
     dv[q].tile0 = tiles;
     tiles += dv[q].ptiles * dv[q].ngroups;
     dv[q].src = (const void* const*)(intptr_t)(0 * nptr_total + pofs);  // patched below
@@ -884,15 +907,16 @@ int adz_plan_autotune(const adz_item* items, int nitems, int fdtype, int math, i
   cur.fields_per_pass = std::min(cur.fields_per_pass, maxf);
   float cur_ms = 0.f;
   int rc = time_one(cur, &cur_ms);
-  const int stages[] = {1, 2, 3}, fpps[] = {2, 4, 8, 16, 32, 64}, bpcs[] = {2, 4, 8}, flags[] = {0, 1};
+  const int stages[] = {1, 2, 3}, fpps[] = {2, 4, 8, 16, 32, 64}, bpcs[] = {2, 4, 8}, flags[] = {0, 1},
+            orders[] = {0, 1, 2, 4, 8};
   // coordinates: 0 staging, 1 tile order, 2 fields per pass, 3 blocks per CU, 4 loop order,
   // 5 XCD chunking; the tile order is searched again after fields per pass (they interact
   // through the L2 working set)
   const int coords[] = {0, 1, 2, 1, 3, 4, 5};
   for (int ci = 0; ci < 7 && rc == ADZ_OK; ci++) {
     const int coord = coords[ci];
-    const int* vals = coord == 0 ? stages : coord == 2 ? fpps : coord == 3 ? bpcs : flags;
-    int nv = coord == 0 ? (fdtype == ADZ_F32 ? 3 : 2) : coord == 2 ? 6 : coord == 3 ? 3 : 2;
+    const int* vals = coord == 0 ? stages : coord == 1 ? orders : coord == 2 ? fpps : coord == 3 ? bpcs : flags;
+    int nv = coord == 0 ? (fdtype == ADZ_F32 ? 3 : 2) : coord == 1 ? 5 : coord == 2 ? 6 : coord == 3 ? 3 : 2;
     for (int v = 0; v < nv && rc == ADZ_OK; v++) {
       adz_tune t = cur;
       int* slot = coord == 0 ? &t.stage : coord == 1 ? &t.level_inner : coord == 2 ? &t.fields_per_pass

@@ -16,6 +16,7 @@
 // how the domain is split.  The weights are recomputed in passes 2 and 3 instead of being
 // stored (w depends only on m, lin, pkps and the sign of D), which saves one array per
 // tracer.  Scalars stay on the device, so a timestep can be captured in a hipGraph.
+#include <algorithm>
 #include <cstring>
 #include <vector>
 
@@ -102,19 +103,25 @@ __device__ __forceinline__ T bc_weight_abs(T m, T lin, double pkps, const BcTr& 
   return (T)v;
 }
 
-// Grid: x = tracer (fastest), y = block of scope rows, so the blocks of every tracer over
-// the same rows are dispatched together and share the air mass, mask and pkps through
-// the caches.  Each block owns whole rows (a fixed partition), lanes stride along i.
+// Each block owns whole rows of one tracer (a fixed partition), lanes stride along i.
 template <class T, int PASS>
 __global__ __launch_bounds__(kBcBlock) void k_bc_pass(BcArgs A) {
-  const int n = blockIdx.x;
+  // 1-D grid of ntr*nblk blocks (a multiple of 8): blocks are dispatched round-robin over
+  // the 8 XCDs, so XCD x takes the x-th eighth of the (row block, tracer) list, tracer
+  // fastest: the blocks of every tracer over the same rows share one XCD's L2 for the air
+  // mass, mask and pkps
+  const long nlog = (long)A.ntr * A.nblk, per = nlog / 8;
+  const long lg = (long)(blockIdx.x % 8) * per + blockIdx.x / 8;
+  if (lg >= nlog) return;
+  const int n = (int)(lg % A.ntr);
+  const int by = (int)(lg / A.ntr);
   const BcTr t = A.tr[n];
   const double* sc = A.sc + n * kSc;
   const bool cyc = (PASS >= P_CORR) && sc[SC_CYCLE] != 0.0;
   const double sd = sgn1(sc[SC_D]);
   const double lambda = sc[SC_LAMBDA], prop = sc[SC_PROP];
   DD s0{0.0, 0.0}, s1{0.0, 0.0}, s2{0.0, 0.0}, s3{0.0, 0.0};
-  const int r0 = (int)((long)A.nrows * blockIdx.y / A.nblk), r1 = (int)((long)A.nrows * (blockIdx.y + 1) / A.nblk);
+  const int r0 = (int)((long)A.nrows * by / A.nblk), r1 = (int)((long)A.nrows * (by + 1) / A.nblk);
   const ADZ_GLOBAL T* air1 = (const ADZ_GLOBAL T*)A.airm1;
   const ADZ_GLOBAL T* air0 = (const ADZ_GLOBAL T*)A.airm0;
   const ADZ_GLOBAL double* maskg = (const ADZ_GLOBAL double*)A.mask;
@@ -212,7 +219,7 @@ __global__ __launch_bounds__(kBcBlock) void k_bc_pass(BcArgs A) {
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    double* o = A.part + ((long)n * A.nblk + blockIdx.y) * 8;
+    double* o = A.part + ((long)n * A.nblk + by) * 8;
 #pragma unroll
     for (int q = 0; q < 2 * NS; q++) o[q] = sh[q][0];
   }
@@ -308,6 +315,7 @@ int adz_bc_create(const adz_bc_tracer* tr, int ntr, const adz_bc_domain* dom, in
   if (nblk > 2048) nblk = 2048;
   if (nblk > a.nrows) nblk = a.nrows;
   if (nblk < 1) nblk = 1;
+  nblk = std::max(8L, nblk / 8 * 8);  // ntr*nblk a multiple of 8 (one eighth per XCD)
   a.nblk = (int)nblk;
   size_t tb = sizeof(BcTr) * ntr, pb = sizeof(double) * 8 * ntr * nblk, sb = sizeof(double) * kSc * ntr;
   void* mem = nullptr;
@@ -340,7 +348,7 @@ int bc_apply(adz_bc* bc, const void* airm1, const void* airm0, const void* pkps,
   a.flux = flux;
   if (bc->has_w3 && !pkps) return fail(ADZ_EINVAL, "adz_bermejo_conde: weight 3 needs pkps");
   hipStream_t st = (hipStream_t)stream;
-  dim3 g((unsigned)a.ntr, (unsigned)a.nblk), b(kBcBlock);
+  dim3 g((unsigned)((long)a.ntr * a.nblk)), b(kBcBlock);
   dim3 gf((unsigned)((a.ntr + 63) / 64)), bf(64);
 #define ADZ_BC_PASSES(T)                                                       \
   hipLaunchKernelGGL((k_bc_pass<T, P_PM>), g, b, 0, st, a);                   \

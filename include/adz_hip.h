@@ -213,8 +213,10 @@ typedef struct adz_tune {
                           3 (FP32 sources): one producer wave per block streams the
                           boxes of several fields ahead into LDS (DMA), four consumer
                           waves evaluate from LDS                                       */
-  int level_inner;     /* 1: consecutive tiles walk the levels of one column (a source
-                          plane feeds 4 levels: vertical reuse in L2); 0: i, j, then k   */
+  int level_inner;     /* tile order. 0: i, j, then k; 1: consecutive tiles walk the
+                          levels of one tile column; S >= 2: super-columns of S tile
+                          rows walked level by level (source planes and rows reused
+                          in L2), then i, then j                                         */
 } adz_tune;
 
 typedef struct adz_plan adz_plan;

@@ -163,7 +163,13 @@ def test_hlt_f32_math_error_vs_fp64_oracle(T):
                                        level_inner=1),
                                   dict(fields_per_pass=5, points_outer=0, blocks_per_cu=3, xcd_chunks=1, stage=3),
                                   dict(fields_per_pass=2, points_outer=1, blocks_per_cu=2, xcd_chunks=0, stage=3,
-                                       level_inner=1)])
+                                       level_inner=1),
+                                  dict(fields_per_pass=4, points_outer=0, blocks_per_cu=4, xcd_chunks=1, stage=2,
+                                       level_inner=4),
+                                  dict(fields_per_pass=5, points_outer=1, blocks_per_cu=2, xcd_chunks=0, stage=1,
+                                       level_inner=3),
+                                  dict(fields_per_pass=3, points_outer=0, blocks_per_cu=3, xcd_chunks=1, stage=3,
+                                       level_inner=2)])
 def test_plan_matches_single_calls(T, tune):
     torch, L, A = T
     cfgs = [S.Config(f"p{s}", 40 + 16 * s, 36 + 8 * s, 16, 4, 300.0, jets=True, cmax_xy=2.0, seed=60 + s)
