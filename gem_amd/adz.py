@@ -21,6 +21,9 @@ L.extra_sigs({
     "adz_ctx_ztab": ("vp", ["vp", "i"]),
     "adz_inittraj": ("i", ["vp", "vp"]),
     "adz_traject": ("i", ["vp", "d", "i", "d", "d"] + ["vp"] * 6 + ["vp"]),
+    "adz_ctx_state_bytes": ("i", ["vp", "vp"]),
+    "adz_ctx_save_state": ("i", ["vp", "vp", "l", "vp"]),
+    "adz_ctx_load_state": ("i", ["vp", "vp", "l", "vp"]),
 })
 
 ADZ_A_PXYZM, ADZ_A_WPXYZ, ADZ_A_UVW_DEP, ADZ_A_PM, ADZ_A_PMU, ADZ_A_PMV, ADZ_A_PT, ADZ_A_PB, ADZ_A_UVW_D, \
@@ -106,6 +109,19 @@ class AdzContext:
             assert a.is_cuda and a.dtype == torch.float32 and a.is_contiguous()
         L.check(L.lib().adz_traject(self.h, float(dt), int(schm_advec), float(bA_m), float(bA),
                                     *[C.c_void_p(a.data_ptr()) for a in args], _stream()), "adz_traject")
+
+    def save_state(self):
+        """The SL state a restart needs (Adz_pxyzm, Adz_wpxyz, the next iteration count) as
+        a host byte array (adz_set.F90:343-360)."""
+        n = C.c_size_t()
+        L.check(L.lib().adz_ctx_state_bytes(self.h, C.byref(n)), "adz_ctx_state_bytes")
+        buf = np.zeros(n.value, np.uint8)
+        L.check(L.lib().adz_ctx_save_state(self.h, buf.ctypes.data, n.value, _stream()), "adz_ctx_save_state")
+        return buf
+
+    def load_state(self, buf):
+        buf = np.ascontiguousarray(buf, dtype=np.uint8)
+        L.check(L.lib().adz_ctx_load_state(self.h, buf.ctypes.data, buf.size, _stream()), "adz_ctx_load_state")
 
     def free(self):
         if getattr(self, "h", None) is not None and self.h.value:

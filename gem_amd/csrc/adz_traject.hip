@@ -569,6 +569,68 @@ int adz_ctx_array(adz_ctx* c, int which, void** dptr, long* nelem) {
 
 const adz_ztab* adz_ctx_ztab(const adz_ctx* c, int thermo) { return c ? (thermo ? c->zth : c->zq) : nullptr; }
 
+// ---------------------------------------------------------------- restart of the SL state
+// What the reference checkpoints for the semi-Lagrangian scheme: ADZ_PXYZM and ADZ_WPXYZ,
+// registered with GMM_FLAG_RSTR (adz_set.F90:343-360).  The blob is a fixed header, then
+// Adz_pxyzm in the context's position storage, then Adz_wpxyz in FP64.  The iteration
+// count of the next adz_traject is kept too (the reference restarts at a step boundary,
+// where it is Adz_itraj: adz_traject.F90:120, adz_set.F90:424).
+namespace {
+struct StateHeader {
+  char magic[8];
+  int32_t pos_dtype, ni, nj, nk, k0m, niter;
+  int64_t n_p3, n_wp;
+};
+constexpr char kStateMagic[8] = {'A', 'D', 'Z', 'S', 'T', 'A', 'T', '1'};
+
+size_t state_bytes(const adz_ctx* c) {
+  const size_t sz = c->pos_dtype == ADZ_F32 ? 4 : 8;
+  return sizeof(StateHeader) + (size_t)c->n_p3 * sz + (size_t)c->n_wp * 8;
+}
+}  // namespace
+
+int adz_ctx_state_bytes(const adz_ctx* c, size_t* bytes) {
+  if (!c || !bytes) return fail(ADZ_EINVAL, "adz_ctx_state_bytes: null argument");
+  *bytes = state_bytes(c);
+  return ADZ_OK;
+}
+
+int adz_ctx_save_state(const adz_ctx* c, void* host, size_t bytes, adz_stream s) {
+  if (!c || !host) return fail(ADZ_EINVAL, "adz_ctx_save_state: null argument");
+  if (bytes < state_bytes(c)) return fail(ADZ_EINVAL, "adz_ctx_save_state: buffer too small");
+  StateHeader h;
+  std::memcpy(h.magic, kStateMagic, 8);
+  h.pos_dtype = c->pos_dtype; h.ni = c->P.ni; h.nj = c->P.nj; h.nk = c->P.nk; h.k0m = c->P.k0m;
+  h.niter = c->niter; h.n_p3 = c->n_p3; h.n_wp = c->n_wp;
+  std::memcpy(host, &h, sizeof(h));
+  char* p = (char*)host + sizeof(h);
+  const size_t b3 = (size_t)c->n_p3 * (c->pos_dtype == ADZ_F32 ? 4 : 8);
+  ADZ_HIP_CHECK(hipMemcpyAsync(p, c->pxyzm, b3, hipMemcpyDeviceToHost, (hipStream_t)s));
+  ADZ_HIP_CHECK(hipMemcpyAsync(p + b3, c->wpxyz, (size_t)c->n_wp * 8, hipMemcpyDeviceToHost, (hipStream_t)s));
+  ADZ_HIP_CHECK(hipStreamSynchronize((hipStream_t)s));
+  return ADZ_OK;
+}
+
+int adz_ctx_load_state(adz_ctx* c, const void* host, size_t bytes, adz_stream s) {
+  if (!c || !host) return fail(ADZ_EINVAL, "adz_ctx_load_state: null argument");
+  if (bytes < sizeof(StateHeader)) return fail(ADZ_EINVAL, "adz_ctx_load_state: buffer too small");
+  StateHeader h;
+  std::memcpy(&h, host, sizeof(h));
+  if (std::memcmp(h.magic, kStateMagic, 8) != 0) return fail(ADZ_EINVAL, "adz_ctx_load_state: not an SL state");
+  if (h.pos_dtype != c->pos_dtype || h.ni != c->P.ni || h.nj != c->P.nj || h.nk != c->P.nk || h.k0m != c->P.k0m ||
+      h.n_p3 != c->n_p3 || h.n_wp != c->n_wp)
+    return fail(ADZ_EINVAL, "adz_ctx_load_state: the state belongs to another grid or position storage");
+  if (bytes < state_bytes(c)) return fail(ADZ_EINVAL, "adz_ctx_load_state: buffer too small");
+  if (h.niter < 1) return fail(ADZ_EINVAL, "adz_ctx_load_state: bad iteration count");
+  const char* p = (const char*)host + sizeof(h);
+  const size_t b3 = (size_t)c->n_p3 * (c->pos_dtype == ADZ_F32 ? 4 : 8);
+  ADZ_HIP_CHECK(hipMemcpyAsync(c->pxyzm, p, b3, hipMemcpyHostToDevice, (hipStream_t)s));
+  ADZ_HIP_CHECK(hipMemcpyAsync(c->wpxyz, p + b3, (size_t)c->n_wp * 8, hipMemcpyHostToDevice, (hipStream_t)s));
+  ADZ_HIP_CHECK(hipStreamSynchronize((hipStream_t)s));
+  c->niter = h.niter;
+  return ADZ_OK;
+}
+
 int adz_inittraj(adz_ctx* c, adz_stream s) {
   if (!c) return fail(ADZ_EINVAL, "adz_inittraj: null context");
   long n = (long)c->P.ni * c->P.nj * (c->P.nk - c->P.k0m + 1);

@@ -171,6 +171,25 @@ module adz_hip
          integer(c_int), value :: thermo
          type(c_ptr) :: lv
       end function
+      ! restart of the SL state (Adz_pxyzm, Adz_wpxyz, next iteration count) via a host buffer
+      function adz_ctx_state_bytes(ctx, bytes) bind(C, name="adz_ctx_state_bytes") result(rc)
+         import :: c_int, c_ptr, c_size_t
+         type(c_ptr), value :: ctx
+         integer(c_size_t), intent(out) :: bytes
+         integer(c_int) :: rc
+      end function
+      function adz_ctx_save_state(ctx, host, bytes, stream) bind(C, name="adz_ctx_save_state") result(rc)
+         import :: c_int, c_ptr, c_size_t
+         type(c_ptr), value :: ctx, host, stream
+         integer(c_size_t), value :: bytes
+         integer(c_int) :: rc
+      end function
+      function adz_ctx_load_state(ctx, host, bytes, stream) bind(C, name="adz_ctx_load_state") result(rc)
+         import :: c_int, c_ptr, c_size_t
+         type(c_ptr), value :: ctx, host, stream
+         integer(c_size_t), value :: bytes
+         integer(c_int) :: rc
+      end function
       function adz_inittraj(ctx, stream) bind(C, name="adz_inittraj") result(rc)
          import :: c_int, c_ptr
          type(c_ptr), value :: ctx, stream

@@ -147,6 +147,20 @@ int adz_ctx_array(adz_ctx* c, int which, void** dptr, long* nelem);
 /* The vertical descriptors adz_set builds (Adz_cpntr_q / Adz_cpntr_t); owned by c. */
 const adz_ztab* adz_ctx_ztab(const adz_ctx* c, int thermo);
 
+/* Restart of the semi-Lagrangian state: the arrays the reference checkpoints for the
+ * scheme, ADZ_PXYZM (Adz_pxyzm, the context's position storage) and ADZ_WPXYZ (Adz_wpxyz,
+ * FP64), registered with GMM_FLAG_RSTR at adz_set.F90:343-360, plus the iteration count of
+ * the next adz_traject.  adz_ctx_save_state copies them into a HOST buffer of
+ * adz_ctx_state_bytes bytes (fixed header, then the arrays), adz_ctx_load_state into a
+ * context built by adz_set for the same grid and position storage; both synchronise the
+ * stream.  A run that saves after step N and continues from a fresh context that loads
+ * the state reproduces the uninterrupted run bit for bit (the reference's restart test).
+ * The BC_mass_deficit of each tracer, which the reference writes in its restart file
+ * (wrrstrt.F90:49-63), is the caller's device array and is saved by the caller. */
+int adz_ctx_state_bytes(const adz_ctx* c, size_t* bytes);
+int adz_ctx_save_state(const adz_ctx* c, void* host, size_t bytes, adz_stream stream);
+int adz_ctx_load_state(adz_ctx* c, const void* host, size_t bytes, adz_stream stream);
+
 /* adz_inittraj (adz_inittraj.F90:16-42): arrival points as first guess, and the next
  * trajectory uses max(6, itraj) iterations. */
 int adz_inittraj(adz_ctx* c, adz_stream stream);
