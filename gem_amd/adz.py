@@ -212,6 +212,27 @@ def adz_tricub_hlt(stack, xyz, geom, i0, in_, j0, jn, k0, nk, dlayout, ext=True,
             "adz_tricub_hlt")
 
 
+L.extra_sigs({"adz_tricub_hlt_qv": ("i", ["vp", "vp", "i", "vp"] + ["i"] * 11 + ["vp"])})
+
+
+def adz_tricub_hlt_qv(ctx, stack, xyz, i0, in_, j0, jn, k0, dlayout, ext=True, post=0, src_off=None):
+    """adz_tricub_hlt(..., F_QV_L=.true.): bicubic horizontal, quintic vertical
+    (bicubHQV_lin_mono.F90:20-172) at the positions xyz (storage of the context) on the
+    thermodynamic levels of ctx; FP32 fields."""
+    dminx, dmaxx, dminy, dmaxy = dlayout
+    dst_off = (1 - dminx) + (dmaxx - dminx + 1) * (1 - dminy)
+    if src_off is None:
+        i = ctx.info
+        src_off = (1 - i["lminx"]) + i["nit"] * (1 - i["lminy"]) if ext else dst_off
+    for e in stack:
+        _check_types(torch.float32, **{k: e.get(k) for k in ("src", "dst", "lin", "min", "max")})
+    _check_types(torch.float32 if ctx.pos_dtype == L.ADZ_F32 else torch.float64, xyz=xyz)
+    fl = field_list(stack, src_off, dst_off)
+    L.check(L.lib().adz_tricub_hlt_qv(ctx.h, fl, len(stack), C.c_void_p(xyz.data_ptr()), i0, in_, j0, jn, k0,
+                                      dminx, dmaxx, dminy, dmaxy, 1 if ext else 0, post, _stream()),
+            "adz_tricub_hlt_qv")
+
+
 def _item_array(items):
     """adz_item array for a list of dicts {geom, xyz, box=(i0,in,j0,jn,k0,nk),
     dlayout=(dminx,dmaxx,dminy,dmaxy), ext, fields=field_list(...)}; returns the array

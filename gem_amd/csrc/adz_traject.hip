@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "adz_common.h"
+#include "adz_qv.h"
 
 using namespace adz;
 
@@ -318,6 +319,7 @@ struct adz_ctx {
   adz_ztab* zq = nullptr;
   adz_ztab* zth = nullptr;
   void* tables = nullptr;  // device copy of the tables
+  void* qvtab = nullptr;   // delz_t and the cubic / quintic denominators (BICHQV)
   // device state
   void* uvw_d = nullptr; void* warr = nullptr; void* uvw_dep = nullptr; void* pxyzm = nullptr;
   double* wpxyz = nullptr;
@@ -355,7 +357,8 @@ double lag3(double zz, double z1, double z2, double z3, double z4) {
 unsigned nb(long n) { return (unsigned)((n + 255) / 256); }
 
 void free_ctx(adz_ctx* c) {
-  void* bufs[] = {c->tables, c->uvw_d, c->warr, c->uvw_dep, c->pxyzm, c->wpxyz, c->pm, c->pmu, c->pmv, c->pt, c->pb};
+  void* bufs[] = {c->tables, c->qvtab, c->uvw_d, c->warr, c->uvw_dep, c->pxyzm, c->wpxyz,
+                  c->pm, c->pmu, c->pmv, c->pt, c->pb};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   if (c->zq) adz_vsearch_free(c->zq);
@@ -499,6 +502,40 @@ int adz_set(const adz_grid* g, int pos_dtype, int itraj, adz_ctx** out) {
   P.search_m = (const int*)d; d += 4 * c->search_m.size();
   P.search_t = (const int*)d;
 
+  // BICHQV tables (adz_set.F90:286-307): delz_t, then the inverse cubic and quintic
+  // Lagrange denominators of the thermodynamic levels, each indexed by k
+  {
+    std::vector<double> q(11 * (size_t)(nk + 1), 0.0);
+    std::memcpy(q.data(), c->delz_t.data(), 8 * (nk + 1));
+    double* c4 = q.data() + (nk + 1);
+    double* c6 = c4 + 4 * (nk + 1);
+    auto tri = [](double za, double zb, double zc, double zd) { return ((za - zb) * (za - zc) * (za - zd)); };
+    auto qui = [](double zx, double za, double zb, double zc, double zd, double ze) {
+      return ((zx - za) * (zx - zb) * (zx - zc) * (zx - zd) * (zx - ze));
+    };
+    for (int k = 2; k <= nk - 2; k++) {
+      double ra = zt[k - 1], rb = zt[k], rc = zt[k + 1], rd = zt[k + 2];
+      c4[k] = 1.0 / tri(ra, rb, rc, rd);
+      c4[(nk + 1) + k] = 1.0 / tri(rb, ra, rc, rd);
+      c4[2 * (nk + 1) + k] = 1.0 / tri(rc, ra, rb, rd);
+      c4[3 * (nk + 1) + k] = 1.0 / tri(rd, ra, rb, rc);
+    }
+    for (int k = 3; k <= nk - 3; k++) {
+      double rx = zt[k - 2], ra = zt[k - 1], rb = zt[k], rc = zt[k + 1], rd = zt[k + 2], re = zt[k + 3];
+      c6[k] = 1.0 / qui(rx, ra, rb, rc, rd, re);
+      c6[(nk + 1) + k] = 1.0 / qui(ra, rx, rb, rc, rd, re);
+      c6[2 * (nk + 1) + k] = 1.0 / qui(rb, rx, ra, rc, rd, re);
+      c6[3 * (nk + 1) + k] = 1.0 / qui(rc, rx, ra, rb, rd, re);
+      c6[4 * (nk + 1) + k] = 1.0 / qui(rd, rx, ra, rb, rc, re);
+      c6[5 * (nk + 1) + k] = 1.0 / qui(re, rx, ra, rb, rc, rd);
+    }
+    if (hipMalloc(&c->qvtab, 8 * q.size()) != hipSuccess) { free_ctx(c); return fail(ADZ_ENOMEM, "adz_set: tables"); }
+    if (hipMemcpy(c->qvtab, q.data(), 8 * q.size(), hipMemcpyHostToDevice) != hipSuccess) {
+      free_ctx(c);
+      return fail(ADZ_EHIP, "adz_set: table upload");
+    }
+  }
+
   // vertical descriptors (adz_set.F90:128-131) over the extended arrays
   int st = adz_vsearch_setup_plus(&zm[1], nk, P.nit, P.njt, 1 - g->i0g, 1 - g->j0g, &c->zq);
   if (st == ADZ_OK) st = adz_vsearch_setup_plus(&zt[1], nk, P.nit, P.njt, 1 - g->i0g, 1 - g->j0g, &c->zth);
@@ -568,6 +605,27 @@ int adz_ctx_array(adz_ctx* c, int which, void** dptr, long* nelem) {
 }
 
 const adz_ztab* adz_ctx_ztab(const adz_ctx* c, int thermo) { return c ? (thermo ? c->zth : c->zq) : nullptr; }
+
+}  // extern "C"
+
+bool adz::ctx_qv(const adz_ctx* c, QvDev* q) {
+  if (!c || !c->qvtab) return false;
+  const int nk = c->P.nk;
+  q->posz = c->P.zt;
+  q->odz = c->P.odelz_t;
+  q->dz = (const double*)c->qvtab;
+  q->c4 = q->dz + (nk + 1);
+  q->c6 = q->c4 + 4 * (nk + 1);
+  q->nk = nk;
+  q->offi = 1 - c->P.i0g;
+  q->offj = 1 - c->P.j0g;
+  q->lminx = c->P.lminx; q->lmaxx = c->P.lminx + c->P.nit - 1;
+  q->lminy = c->P.lminy; q->lmaxy = c->P.lminy + c->P.njt - 1;
+  q->pos_dtype = c->pos_dtype;
+  return true;
+}
+
+extern "C" {
 
 // ---------------------------------------------------------------- restart of the SL state
 // What the reference checkpoints for the semi-Lagrangian scheme: ADZ_PXYZM and ADZ_WPXYZ,

@@ -4,7 +4,8 @@
 !   adz_set -> adz_inittraj -> adz_traject(dt) -> adz_tricub_hlt(tracers at Adz_pt, with
 !   Adz_post, with the LAM boundary flux) -> adz_post_tr(1) -> adz_post_tr(0) (Bermejo-Conde)
 !
-! Usage: adz_f_driver <input file> <output file>
+! Usage: adz_f_driver <input file> <output file> [qv]
+!        (qv: the tracers with quintic vertical interpolation, F_QV_L = .true.)
 !        adz_f_driver sizes      (prints the byte sizes of the bind(C) types)
 ! The input is a flat little-endian stream (written by the caller):
 !   int32  ni nj nk halox haloy pil itraj nfld mh pos_dtype
@@ -24,7 +25,7 @@ program adz_f_driver
                       hipMemcpyHostToDevice, hipMemcpyDeviceToHost
    use adz_gpu
    implicit none
-   character(len=512) :: fin, fout
+   character(len=512) :: fin, fout, mode
    integer(c_int) :: hdr(10)
    real(c_double) :: dbl(8)
    integer :: ni, nj, nk, hx, hy, pil, itraj, nfld, mh, pdt, nit, njt, n, u
@@ -40,6 +41,7 @@ program adz_f_driver
 
    call get_command_argument(1, fin)
    call get_command_argument(2, fout)
+   call get_command_argument(3, mode)
    if (trim(fin) == 'sizes') then
       call print_sizes()
       stop
@@ -102,7 +104,7 @@ program adz_f_driver
    call adz_inittraj()
    call adz_traject(dbl(8))
    call adz_tricub_hlt(Adz_stack, nfld, Adz_pt, Adz_cpntr_t, Adz_num_t, Adz_i0, Adz_in, Adz_j0, Adz_jn, &
-                       Adz_k0t, .false., Tr_3CWP)
+                       Adz_k0t, .false., Tr_3CWP, F_QV_L=trim(mode) == 'qv')
    call adz_post_tr(1)
    call adz_post_tr(0)
    if (hipDeviceSynchronize() /= 0) error stop 'hipDeviceSynchronize'

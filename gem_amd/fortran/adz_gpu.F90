@@ -23,9 +23,9 @@
 !   * With F_post present, tracers whose meta has mono == 1 are clipped to [min, max]
 !     inside the interpolation launch (the same operation adz_post_tr.F90:81-94 applies to
 !     the same box afterwards), so adz_post_tr(1) only registers the fixer's tracers.
-!   * ILMC (mono = 2), quintic vertical (F_QV_L), the BICHQV group (F_wp = 2) and the ZLF
-!     boundary flux (Adz_BC_LAM_flux = 2) are not part of this library: they stop with a
-!     message.
+!   * ILMC (mono = 2), the fixer of the BICHQV group (adz_post_tr(2)) and the ZLF boundary
+!     flux (Adz_BC_LAM_flux = 2) are not part of this library: they stop with a message.
+!     Quintic vertical interpolation (F_QV_L) is (FP32 fields).
 !   * Bermejo_Conde synchronises the device to return each tracer's mass_deficit in
 !     F_bc(n)%mass_deficit, as the reference's host scalar.
 !   * The LAM boundary flux keeps only what Bermejo_Conde reads of it: the masses of
@@ -175,14 +175,14 @@ contains
       type(c_ptr), intent(in) :: F_geom
       type(adz_field), allocatable :: fl(:)
       integer :: n, pass, cnt, post_flag
-      logical :: ext_L
+      logical :: ext_L, qv_L
       ext_L = .true.
       if (present(F_ext_L)) ext_L = F_ext_L
-      if (present(F_QV_L)) then
-         if (F_QV_L) then
-            write (error_unit, *) 'adz_tricub_hlt: quintic vertical interpolation is not part of this library'
-            error stop 1
-         end if
+      qv_L = .false.
+      if (present(F_QV_L)) qv_L = F_QV_L
+      if (qv_L .and. ext_L) then  ! adz_interp_hlt.F90:63-66
+         write (error_unit, *) 'Source must be NOT expended when using quintic - abort'
+         error stop 1
       end if
       if ((F_in - F_i0 + 1) <= 0 .or. (F_jn - F_j0 + 1) <= 0) return
       ! adz_interp_hlt.F90:91-96: the LAM boundary flux of the fixer's tracers
@@ -218,6 +218,14 @@ contains
          post_flag = 0
          if (present(F_post)) post_flag = ADZ_POST_STORE
          if (pass == 2) post_flag = ior(post_flag, ADZ_POST_CLIP)
+         if (qv_L) then  ! bicubHQV_lin_min_max_ntr on the thermodynamic levels (adz_interp_hlt.F90:113-115)
+            call adz_check(adz_tricub_hlt_qv(Adz_ctx, fl, int(cnt, c_int), F_xyz, int(F_i0, c_int), &
+                                             int(F_in, c_int), int(F_j0, c_int), int(F_jn, c_int), &
+                                             int(F_k0, c_int), int(l_minx, c_int), int(l_maxx, c_int), &
+                                             int(l_miny, c_int), int(l_maxy, c_int), 0_c_int, &
+                                             int(post_flag, c_int), Adz_stream), 'adz_tricub_hlt (F_QV_L)')
+            cycle
+         end if
          call adz_check(c_adz_tricub_hlt(fl, int(cnt, c_int), F_xyz, int(Adz_pos_dtype, c_int), F_geom, &
                                          int(F_i0, c_int), int(F_in, c_int), int(F_j0, c_int), int(F_jn, c_int), &
                                          int(F_k0, c_int), int(l_nk, c_int), int(l_minx, c_int), &

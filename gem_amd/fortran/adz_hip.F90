@@ -214,6 +214,14 @@ module adz_hip
                                   fdtype, math
          integer(c_int) :: rc
       end function
+      function adz_tricub_hlt_qv(ctx, fld, nptr, xyz, i0, in, j0, jn, k0, dminx, dmaxx, dminy, dmaxy, ext, post, &
+                                 stream) bind(C, name="adz_tricub_hlt_qv") result(rc)
+         import :: c_int, c_ptr, adz_field
+         type(c_ptr), value :: ctx, xyz, stream
+         type(adz_field), intent(in) :: fld(*)
+         integer(c_int), value :: nptr, i0, in, j0, jn, k0, dminx, dmaxx, dminy, dmaxy, ext, post
+         integer(c_int) :: rc
+      end function
       function adz_plan_create(items, nitems, fdtype, math, post, tune, plan) bind(C, name="adz_plan_create") &
             result(rc)
          import :: c_int, c_ptr, adz_item

@@ -202,6 +202,17 @@ int adz_tricub_hlt(const adz_field* fld, int nptr, const void* xyz, int pdtype, 
                    int in, int j0, int jn, int k0, int nk, int dminx, int dmaxx, int dminy, int dmaxy, int ext,
                    int post, int fdtype, int math, adz_stream stream);
 
+/* adz_tricub_hlt with F_QV_L = .true. (adz_interp_hlt.F90:59-66,113-115): bicubic in x and
+ * y, quintic in z (cubic at kk1 = 2 and nk-2, linear in the end intervals), with the
+ * linear result and min / max of the inner 2x2x2 box, as bicubHQV_lin_min_max_ntr and
+ * bicubHQV_lin_min_max_zyx compute them (bicubHQV_lin_mono.F90:20-172,
+ * bicubHQV_lin_min_max_zyx.F90:20-99).  Positions have the context's storage type and
+ * index the thermodynamic levels of ctx (Adz_pt); fields are FP32 (`real`); ext and post as
+ * for adz_tricub_hlt.  The reference's arithmetic: FP64 weights and sums, FP32 row
+ * results, no contraction. */
+int adz_tricub_hlt_qv(adz_ctx* ctx, const adz_field* fld, int nptr, const void* xyz, int i0, int in, int j0, int jn,
+                      int k0, int dminx, int dmaxx, int dminy, int dmaxy, int ext, int post, adz_stream stream);
+
 /* A batched work item: one subdomain (or position set) with its own descriptor,
  * positions and field group (the arguments of one adz_tricub_hlt call). */
 typedef struct adz_item {

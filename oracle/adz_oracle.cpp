@@ -238,3 +238,47 @@ extern "C" double ora_mass(int tdt, const int* dom, const void* x, const void* a
   return mass_of(D, (const double*)x, (const double*)air);
 }
 
+
+// -------------------------------- bicubic H / quintic V (BICHQV) --------------------------
+#include "oracle_qv.h"
+
+extern "C" void* ora_qv_new(const double* zt, int nk) { return new QvTab(qv_setup(zt, nk)); }
+extern "C" void ora_qv_free(void* h) { delete (QvTab*)h; }
+
+// bicubHQV_lin_min_max_ntr (bicubHQV_lin_mono.F90:20-172): fs = array starts (minx, miny, 1)
+// of the nf fields; dom = minx maxx miny maxy ioff joff; outputs [f*n + p].
+template <class TP>
+static void qv_run(const QvTab& T, const float* const* fs, int nf, const TP* pos, long n, const int* dom, float* d,
+                   float* l, float* mi, float* ma) {
+  const long nit = dom[1] - dom[0] + 1, nij = nit * (dom[3] - dom[2] + 1);
+#pragma omp parallel for schedule(static)
+  for (long p = 0; p < n; p++) {
+    double cx[4], cy[4], qz[6], lx[2], ly[2], lz[2];
+    bool zq, zc;
+    long o = qv_coef<TP>(pos[3 * p], pos[3 * p + 1], pos[3 * p + 2], T, dom[0], dom[2], nit, nij, dom[4], dom[5], cx,
+                         cy, qz, lx, ly, lz, &zq, &zc);
+    for (int f = 0; f < nf; f++)
+      qv_zyx(d + (long)f * n + p, l + (long)f * n + p, mi + (long)f * n + p, ma + (long)f * n + p, fs[f] + o, cx, cy,
+             qz, lx, ly, lz, nit, nij, zq, zc);
+  }
+}
+extern "C" void ora_qv_m_n(const void* h, const void* const* fs, int nf, const void* pos, int pdt, long n,
+                           const int* dom, float* d, float* l, float* mi, float* ma) {
+  const QvTab& T = *(const QvTab*)h;
+  if (pdt == 1) qv_run<float>(T, (const float* const*)fs, nf, (const float*)pos, n, dom, d, l, mi, ma);
+  else qv_run<double>(T, (const float* const*)fs, nf, (const double*)pos, n, dom, d, l, mi, ma);
+}
+// the weights of one target (for the pins): cx cy qz lx ly lz (20 doubles), flags, offset
+extern "C" long ora_qv_coef(const void* h, const float* pos, const int* dom, double* w, int* flags) {
+  const QvTab& T = *(const QvTab*)h;
+  const long nit = dom[1] - dom[0] + 1, nij = nit * (dom[3] - dom[2] + 1);
+  bool zq, zc;
+  long o = qv_coef<float>(pos[0], pos[1], pos[2], T, dom[0], dom[2], nit, nij, dom[4], dom[5], w, w + 4, w + 8,
+                          w + 14, w + 16, w + 18, &zq, &zc);
+  flags[0] = zq; flags[1] = zc;
+  return o;
+}
+extern "C" void ora_qv_zyx(float* out4, const float* src, const double* w, long ni, long ninj, int zq, int zc) {
+  qv_zyx(out4, out4 + 1, out4 + 2, out4 + 3, src, w, w + 4, w + 8, w + 14, w + 16, w + 18, ni, ninj, zq != 0,
+         zc != 0);
+}

@@ -17,6 +17,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "liboracle.so")
 REF_AVX2 = os.path.join(HERE, "_ref", "libtricublin_ref_avx2.so")
 REF_SCALAR = os.path.join(HERE, "_ref", "libtricublin_ref_scalar.so")
+REF_QVZYX = os.path.join(HERE, "_ref", "libqvzyx_ref.so")
 
 ORD_AVX2, ORD_SCALAR = 0, 1
 _lib = None
@@ -56,6 +57,13 @@ def lib():
         L.ora_bermejo_conde.argtypes = [i, vp, i, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]
         L.ora_mass.restype = d
         L.ora_mass.argtypes = [i, vp, vp, vp, vp]
+        L.ora_qv_new.restype = vp
+        L.ora_qv_new.argtypes = [vp, i]
+        L.ora_qv_free.argtypes = [vp]
+        L.ora_qv_m_n.argtypes = [vp, vp, i, vp, i, l, vp, vp, vp, vp, vp]
+        L.ora_qv_coef.restype = l
+        L.ora_qv_coef.argtypes = [vp, vp, vp, vp, vp]
+        L.ora_qv_zyx.argtypes = [vp, vp, vp, l, l, i, i]
         _lib = L
     return _lib
 
@@ -432,3 +440,73 @@ def flux_masses(fo, fi, airm0, mask, layout, bbox, sum_mode=1):
     for o, i_ in zip(fo, fi):
         out += [mass(o, airm0, mask, layout, scope, sum_mode), mass(i_, airm0, mask, layout, scope, sum_mode)]
     return np.array(out)
+
+
+# ------------------------------------------------------------------ bicubic H / quintic V
+class QvTab:
+    """Tables of the BICHQV interpolation (adz_set.F90:181-190,286-307); zt = Ver_z_8%t(0:nk+1)."""
+
+    def __init__(self, zt, nk):
+        self.zt = np.ascontiguousarray(zt, dtype=np.float64)
+        self.nk = nk
+        self.h = lib().ora_qv_new(_p(self.zt), nk)
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.ora_qv_free(self.h)
+            self.h = None
+
+
+def qv_m_n(tab, fields, layout, pos, ioff, joff):
+    """bicubHQV_lin_min_max_ntr (bicubHQV_lin_mono.F90:20-172).  fields: float32 flat arrays
+    starting at (minx, miny, 1) of layout = (minx, maxx, miny, maxy); ioff, joff = Adz_ioff,
+    Adz_joff (l_i0 - 2, l_j0 - 2).  Returns (qut, lin, min, max), each (nfields, n)."""
+    pos = np.ascontiguousarray(pos)
+    n = pos.size // 3
+    outs = [np.zeros(len(fields) * n, np.float32) for _ in range(4)]
+    fs = _ptrs(fields, [0] * len(fields))
+    dom = np.array(list(layout) + [ioff, joff], np.int32)
+    lib().ora_qv_m_n(tab.h, fs, len(fields), _p(pos), _pdt(pos), n, _p(dom), *[_p(o) for o in outs])
+    return [o.reshape(len(fields), n) for o in outs]
+
+
+def qv_coef(tab, pos3, layout, ioff, joff):
+    """Weights of one target: (w[20] = cx cy qz lx ly lz, quintic, cubic, element offset)."""
+    p = np.ascontiguousarray(pos3, dtype=np.float32)
+    w = np.zeros(20)
+    fl = np.zeros(2, np.int32)
+    dom = np.array(list(layout) + [ioff, joff], np.int32)
+    off = lib().ora_qv_coef(tab.h, _p(p), _p(dom), _p(w), _p(fl))
+    return w, bool(fl[0]), bool(fl[1]), off
+
+
+def qv_zyx(src, off, w, ni, ninj, zq, zc):
+    """bicubHQV_lin_min_max_zyx restated: (qut, lin, min, max) of the stencil at src[off]."""
+    out = np.zeros(4, np.float32)
+    lib().ora_qv_zyx(_p(out), C.c_void_p(src.ctypes.data + 4 * int(off)), _p(np.ascontiguousarray(w)), ni, ninj,
+                     int(zq), int(zc))
+    return out
+
+
+class RefQvZyx:
+    """The reference's bicubHQV_lin_min_max_zyx compiled from its own source (oracle/_ref)."""
+
+    def __init__(self):
+        if not os.path.exists(REF_QVZYX):
+            raise FileNotFoundError(REF_QVZYX)
+        self.L = C.CDLL(REF_QVZYX)
+        self.f = self.L.bicubhqv_lin_min_max_zyx_
+        self.f.argtypes = [C.c_void_p] * 15
+
+    def __call__(self, src, off, w, ni, ninj, zq, zc):
+        out = np.zeros(4, np.float32)
+        w = np.ascontiguousarray(w, dtype=np.float64)
+        ints = np.array([ni, ninj, 1 if zq else 0, 1 if zc else 0], np.int32)
+        a = out.ctypes.data
+        wd = w.ctypes.data
+        self.f(C.c_void_p(a), C.c_void_p(a + 4), C.c_void_p(a + 8), C.c_void_p(a + 12),
+               C.c_void_p(src.ctypes.data + 4 * int(off)), C.c_void_p(wd), C.c_void_p(wd + 32),
+               C.c_void_p(wd + 64), C.c_void_p(wd + 112), C.c_void_p(wd + 128), C.c_void_p(wd + 144),
+               C.c_void_p(ints.ctypes.data), C.c_void_p(ints.ctypes.data + 4), C.c_void_p(ints.ctypes.data + 8),
+               C.c_void_p(ints.ctypes.data + 12))
+        return out

@@ -4,7 +4,8 @@ adz_inittraj, adz_traject(F_dt_8), adz_tricub_hlt(F_stk, ..., F_post), adz_post_
 adz_post_tr(0) -> Bermejo_Conde -- and its results are bit-identical to the same step
 called through the C ABI from Python: departure positions, interpolated and clipped
 tracers, Adz_post lin/min/max, fixed tracers and the carried mass deficits (the fixer with
-the LAM boundary flux, Adz_BC_LAM_flux = 1)."""
+the LAM boundary flux, Adz_BC_LAM_flux = 1).  Both interpolation forms: tricubic, and
+bicubic-horizontal / quintic-vertical (F_QV_L = .true.)."""
 import os
 import subprocess
 
@@ -19,7 +20,8 @@ DRIVER = os.path.join(ROOT, "gem_amd", "fortran", "adz_f_driver")
 MH = 2
 
 
-def test_fortran_step_matches_c_abi(cuda, tmp_path):
+@pytest.mark.parametrize("form", ["tricubic", "quintic"])
+def test_fortran_step_matches_c_abi(cuda, tmp_path, form):
     if not os.path.exists(DRIVER):
         pytest.fail("gem_amd/fortran/adz_f_driver is not built")
     import torch
@@ -61,7 +63,8 @@ def test_fortran_step_matches_c_abi(cuda, tmp_path):
         mass.tofile(f)
         mono.tofile(f)
     print("[gpu-case] running the Fortran driver", flush=True)
-    r = subprocess.run([DRIVER, str(fin), str(fout)], capture_output=True, text=True, timeout=120)
+    r = subprocess.run([DRIVER, str(fin), str(fout)] + (["qv"] if form == "quintic" else []), capture_output=True,
+                       text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
 
     # the same step through the C ABI
@@ -79,8 +82,13 @@ def test_fortran_step_matches_c_abi(cuda, tmp_path):
     for clip in (0, 1):
         grp = [e for e, m in zip(ent, mono) if (m == 1) == bool(clip)]
         post = A.ADZ_POST_STORE | (A.ADZ_POST_CLIP if clip else 0)
-        A.adz_tricub_hlt(grp, ctx.array(A.ADZ_A_PT), ctx.ztab(True), *box[:5], box[5], dl, ext=False, post=post,
-                         math=L.ADZ_MATH_REF, src_off=off)
+        if not grp:
+            continue
+        if form == "quintic":
+            A.adz_tricub_hlt_qv(ctx, grp, ctx.array(A.ADZ_A_PT), *box[:5], dl, ext=False, post=post, src_off=off)
+        else:
+            A.adz_tricub_hlt(grp, ctx.array(A.ADZ_A_PT), ctx.ztab(True), *box[:5], box[5], dl, ext=False,
+                             post=post, math=L.ADZ_MATH_REF, src_off=off)
     bcn = [n for n in range(4) if mass[n] == 1 or 111 <= mass[n] <= 139]
     bcs = [dict(p=ent[n]["src"], m=ent[n]["dst"], lin=ent[n]["lin"], min=ent[n]["min"], max=ent[n]["max"],
                 mass=int(mass[n])) for n in bcn]
