@@ -451,3 +451,23 @@ class BcLamFlux:
             self.free()
         except Exception:
             pass
+
+
+# ---------------------------------------------------------------------- ZLF
+L.extra_sigs({"adz_bc_lam_zlf": ("i", ["i", "vp", "vp", "i", "vp", "i", "vp"])})
+
+
+class AdzZlfDomain(C.Structure):
+    _fields_ = [(n, C.c_int) for n in ("minx", "maxx", "miny", "maxy", "nk", "ni", "nj", "pil_w", "pil_e", "pil_s",
+                                       "pil_n", "ext", "k0")]
+
+
+def adz_bc_lam_zlf(setup, f1, f2, layout, tile, pil, ext, k0, off):
+    """adz_BC_LAM_zlf_hlt(F1, F2, ..., F_setup) (adz_BC_LAM_zlf_hlt.F90:18-319) on lists of
+    device tensors (model layout, element (1,1,1) at offset `off`); f2 may be None for
+    setups 1 and 3.  pil = (pil_w, pil_e, pil_s, pil_n); ext = Grd_maxcfl + 1; k0 = Adz_k0t."""
+    nf = len(f1)
+    a1 = (C.c_void_p * nf)(*[_ptr_at(t, off) for t in f1])
+    a2 = (C.c_void_p * nf)(*[_ptr_at(t, off) for t in f2]) if f2 is not None else None
+    dom = AdzZlfDomain(*layout, *tile, *pil, ext, k0)
+    L.check(L.lib().adz_bc_lam_zlf(setup, a1, a2, nf, C.byref(dom), _DT[f1[0].dtype], _stream()), "adz_bc_lam_zlf")

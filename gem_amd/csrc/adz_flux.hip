@@ -408,3 +408,93 @@ int adz_flux_free(adz_flux* fx) {
 }
 
 }  // extern "C"
+
+// ====================================================================== ZLF
+// The ZLF form of the fixer's LAM boundary treatment (Adz_BC_LAM_flux = 2):
+// adz_BC_LAM_zlf_hlt.F90:18-319, four element-wise steps on model-layout arrays of a tile
+// whose four sides are LAM edges.  Every element of the tile 1..ni x 1..nj x 1..nk is
+// visited once; the halo is not touched (the reference's loops cover the tile only).
+//   setup 1 (F1 = source): zero outside the extension E = [1+pil_w-ext, ni-pil_e+ext] x
+//           [1+pil_s-ext, nj-pil_n+ext], and inside E below level nint(k0/2);
+//   setup 2 (F1 = destination, F2 = piloting copy): F2 = F1 outside the core
+//           C = [1+pil_w, ni-pil_e] x [1+pil_s, nj-pil_n] x [k0, nk];
+//   setup 3 (F1 = destination): zero outside [1+4, ni-4] x [1+4, nj-4] (BCS_BASE = 4);
+//   setup 4 (F1 = destination, F2 = piloting copy): F1 = F2 outside C.
+namespace adz {
+
+constexpr int kZlfFields = 16;
+
+struct ZlfArgs {
+  int setup, ni, nj, nk, nxa, nya;
+  int i0e, ine, j0e, jne, k0d2;   // setup 1
+  int i0c, inc, j0c, jnc, k0;     // setups 2 and 4
+  int i0b, inb, j0b, jnb;         // setup 3
+  int nf;
+  void* f1[kZlfFields];
+  void* f2[kZlfFields];
+};
+
+template <class T>
+__global__ __launch_bounds__(256) void k_zlf(ZlfArgs A) {
+  const long p = (long)blockIdx.x * 256 + threadIdx.x;
+  const long n = (long)A.ni * A.nj * A.nk;
+  if (p >= n) return;
+  const int i = 1 + (int)(p % A.ni);
+  const long r = p / A.ni;
+  const int j = 1 + (int)(r % A.nj), k = 1 + (int)(r / A.nj);
+  const long q = (long)(i - 1) + (long)A.nxa * ((j - 1) + (long)A.nya * (k - 1));
+  bool act;
+  if (A.setup == 1) {
+    const bool inE = i >= A.i0e && i <= A.ine && j >= A.j0e && j <= A.jne;
+    act = !inE || k < A.k0d2;
+  } else if (A.setup == 3) {
+    act = !(i >= A.i0b && i <= A.inb && j >= A.j0b && j <= A.jnb);
+  } else {
+    act = !(i >= A.i0c && i <= A.inc && j >= A.j0c && j <= A.jnc && k >= A.k0);
+  }
+  if (!act) return;
+  for (int f = 0; f < A.nf; f++) {
+    ADZ_GLOBAL T* a = (ADZ_GLOBAL T*)A.f1[f];
+    ADZ_GLOBAL T* b = (ADZ_GLOBAL T*)A.f2[f];
+    if (A.setup == 1 || A.setup == 3) a[q] = (T)0;
+    else if (A.setup == 2) b[q] = a[q];
+    else a[q] = b[q];
+  }
+}
+
+}  // namespace adz
+
+extern "C" int adz_bc_lam_zlf(int setup, void* const* f1, void* const* f2, int nf, const adz_zlf_domain* dom,
+                              int dtype, adz_stream stream) {
+  if (setup < 1 || setup > 4) return fail(ADZ_EINVAL, "ADZ_BC_LAM_ZLF: SETUP not defined");
+  if (!f1 || !dom || nf < 0) return fail(ADZ_EINVAL, "adz_bc_lam_zlf: bad argument");
+  if ((setup == 2 || setup == 4) && !f2) return fail(ADZ_EINVAL, "adz_bc_lam_zlf: setups 2 and 4 need F2");
+  if (dtype != ADZ_F32 && dtype != ADZ_F64) return fail(ADZ_EINVAL, "adz_bc_lam_zlf: dtype must be F32 or F64");
+  const adz_zlf_domain& d = *dom;
+  if (d.ni < 1 || d.nj < 1 || d.nk < 1 || d.minx > 1 || d.miny > 1 || d.maxx < d.ni || d.maxy < d.nj)
+    return fail(ADZ_EINVAL, "adz_bc_lam_zlf: tile outside the arrays");
+  ZlfArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.setup = setup; a.ni = d.ni; a.nj = d.nj; a.nk = d.nk;
+  a.nxa = d.maxx - d.minx + 1; a.nya = d.maxy - d.miny + 1;
+  a.i0e = 1 + d.pil_w - d.ext; a.ine = d.ni - d.pil_e + d.ext;
+  a.j0e = 1 + d.pil_s - d.ext; a.jne = d.nj - d.pil_n + d.ext;
+  a.k0d2 = (int)std::lround(d.k0 / 2.0);  // nint(k0/2.): halves round away from zero
+  a.i0c = 1 + d.pil_w; a.inc = d.ni - d.pil_e; a.j0c = 1 + d.pil_s; a.jnc = d.nj - d.pil_n; a.k0 = d.k0;
+  a.i0b = 1 + 4; a.inb = d.ni - 4; a.j0b = 1 + 4; a.jnb = d.nj - 4;
+  const long n = (long)d.ni * d.nj * d.nk;
+  for (int f0 = 0; f0 < nf; f0 += kZlfFields) {
+    a.nf = std::min(kZlfFields, nf - f0);
+    for (int f = 0; f < a.nf; f++) {
+      a.f1[f] = f1[f0 + f];
+      a.f2[f] = f2 ? f2[f0 + f] : nullptr;
+      if (!a.f1[f] || ((setup == 2 || setup == 4) && !a.f2[f]))
+        return fail(ADZ_EINVAL, "adz_bc_lam_zlf: null array");
+    }
+    const dim3 g((unsigned)((n + 255) / 256)), b(256);
+    if (dtype == ADZ_F32) hipLaunchKernelGGL((k_zlf<float>), g, b, 0, (hipStream_t)stream, a);
+    else hipLaunchKernelGGL((k_zlf<double>), g, b, 0, (hipStream_t)stream, a);
+  }
+  ADZ_HIP_CHECK(hipGetLastError());
+  return ADZ_OK;
+}

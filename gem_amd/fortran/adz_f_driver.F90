@@ -4,8 +4,11 @@
 !   adz_set -> adz_inittraj -> adz_traject(dt) -> adz_tricub_hlt(tracers at Adz_pt, with
 !   Adz_post, with the LAM boundary flux) -> adz_post_tr(1) -> adz_post_tr(0) (Bermejo-Conde)
 !
-! Usage: adz_f_driver <input file> <output file> [qv]
-!        (qv: the tracers with quintic vertical interpolation, F_QV_L = .true.)
+! Usage: adz_f_driver <input file> <output file> [qv|zlf]
+!        (qv: the tracers with quintic vertical interpolation, F_QV_L = .true.;
+!         zlf: the ZLF boundary treatment, Adz_BC_LAM_flux = 2, as adz_tracers.F90:88-99,
+!         154-163 runs it: interpolation on Adz_pb, the piloting values of the
+!         destinations saved before and restored after the fixer)
 !        adz_f_driver sizes      (prints the byte sizes of the bind(C) types)
 ! The input is a flat little-endian stream (written by the caller):
 !   int32  ni nj nk halox haloy pil itraj nfld mh pos_dtype
@@ -36,7 +39,7 @@ program adz_f_driver
    real(c_double), allocatable, target :: posd(:)
    integer(c_int), allocatable :: mass(:), mono(:)
    type(adz_grid), target :: g
-   type(c_ptr), allocatable :: d_src(:), d_dst(:), d_lin(:), d_min(:), d_max(:)
+   type(c_ptr), allocatable :: d_src(:), d_dst(:), d_lin(:), d_min(:), d_max(:), d_pil(:)
    real(kind=REAL64), allocatable :: deficits(:)
 
    call get_command_argument(1, fin)
@@ -58,6 +61,7 @@ program adz_f_driver
    g%ni = ni; g%nj = nj; g%nk = nk; g%i0g = 1; g%j0g = 1; g%halox = hx; g%haloy = hy
    g%west = 1; g%east = 1; g%south = 1; g%north = 1
    g%pil_w = pil; g%pil_e = pil; g%pil_s = pil; g%pil_n = pil; g%gbpil_t = 0; g%bc_lam_flux = 1
+   if (trim(mode) == 'zlf') g%bc_lam_flux = 2
    g%niu = ni - 1; g%njv = nj - 1
    g%inv_hx = dbl(1); g%inv_hy = dbl(2); g%inv_rayt = dbl(3)
    g%zm = c_loc(zm); g%zt = c_loc(zt); g%zx = c_loc(zx); g%lat = c_loc(lat)
@@ -74,12 +78,12 @@ program adz_f_driver
    Adz_uu_ext = upload_f32(n_ext); Adz_vv_ext = upload_f32(n_ext); Adz_ww_ext = upload_f32(n_ext)
    ut0 = upload_f32(n_ext); vt0 = upload_f32(n_ext); zdt0 = upload_f32(n_ext)
 
-   allocate (d_src(nfld), d_dst(nfld), d_lin(nfld), d_min(nfld), d_max(nfld))
+   allocate (d_src(nfld), d_dst(nfld), d_lin(nfld), d_min(nfld), d_max(nfld), d_pil(nfld))
    allocate (Adz_stack(nfld), Adz_post(nfld), Tr_3CWP(nfld), mass(nfld), mono(nfld), deficits(nfld))
    do n = 1, nfld
       d_src(n) = upload_f32(n_mod)
       d_dst(n) = zeros_f32(n_mod); d_lin(n) = zeros_f32(n_mod)
-      d_min(n) = zeros_f32(n_mod); d_max(n) = zeros_f32(n_mod)
+      d_min(n) = zeros_f32(n_mod); d_max(n) = zeros_f32(n_mod); d_pil(n) = zeros_f32(n_mod)
    end do
    airm1 = upload_f32(n_mod); airm0 = upload_f32(n_mod); pkps = upload_f32(n_mod)
    allocate (mask(n_mask))
@@ -95,6 +99,7 @@ program adz_f_driver
       Adz_post(n)%lin = offset(d_lin(n))
       Adz_post(n)%min = offset(d_min(n))
       Adz_post(n)%max = offset(d_max(n))
+      Adz_stack(n)%pil = offset(d_pil(n))
       Tr_3CWP(n)%mass = mass(n)
       Tr_3CWP(n)%mono = mono(n)
    end do
@@ -103,10 +108,18 @@ program adz_f_driver
 
    call adz_inittraj()
    call adz_traject(dbl(8))
-   call adz_tricub_hlt(Adz_stack, nfld, Adz_pt, Adz_cpntr_t, Adz_num_t, Adz_i0, Adz_in, Adz_j0, Adz_jn, &
-                       Adz_k0t, .false., Tr_3CWP, F_QV_L=trim(mode) == 'qv')
+   if (trim(mode) == 'zlf') then
+      call adz_BC_LAM_zlf_0_hlt(nfld, 0)
+      call adz_tricub_hlt(Adz_stack, nfld, Adz_pb, Adz_cpntr_t, Adz_num_b, Adz_i0b, Adz_inb, Adz_j0b, Adz_jnb, &
+                          1, .false., Tr_3CWP)
+      call adz_BC_LAM_zlf_0_hlt(nfld, 1)
+   else
+      call adz_tricub_hlt(Adz_stack, nfld, Adz_pt, Adz_cpntr_t, Adz_num_t, Adz_i0, Adz_in, Adz_j0, Adz_jn, &
+                          Adz_k0t, .false., Tr_3CWP, F_QV_L=trim(mode) == 'qv')
+   end if
    call adz_post_tr(1)
    call adz_post_tr(0)
+   if (trim(mode) == 'zlf') call adz_BC_LAM_zlf_0_hlt(nfld, 2)
    if (hipDeviceSynchronize() /= 0) error stop 'hipDeviceSynchronize'
 
    open (newunit=u, file=trim(fout), access='stream', form='unformatted', status='replace')

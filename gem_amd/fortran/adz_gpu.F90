@@ -11,6 +11,8 @@
 !   Bermejo_Conde.F90:18-320    -> Bermejo_Conde(F_bc, F_ntr_bc, F_i0, F_in, F_j0, F_jn, F_k0)
 !   adz_BC_LAM_Aranami_hlt.F90:18-236 -> adz_BC_LAM_Aranami_hlt(F_stk, F_post, F_nptr), called by
 !                                    adz_tricub_hlt with F_post on a LAM with Adz_BC_LAM_flux = 1
+!   adz_BC_LAM_zlf_hlt.F90:18-319, adz_BC_LAM_zlf_0_hlt.F90:18-102 -> adz_BC_LAM_zlf_hlt(F1, F2, F_n,
+!                                    F_setup), adz_BC_LAM_zlf_0_hlt(F_nptr, F_number)
 !
 ! The module variables stand in for what the reference keeps in adz_mem / adz_options /
 ! dyn_fisl_options / cstv / tr3d (same names).  Array-valued members that the reference
@@ -23,9 +25,10 @@
 !   * With F_post present, tracers whose meta has mono == 1 are clipped to [min, max]
 !     inside the interpolation launch (the same operation adz_post_tr.F90:81-94 applies to
 !     the same box afterwards), so adz_post_tr(1) only registers the fixer's tracers.
-!   * ILMC (mono = 2), the fixer of the BICHQV group (adz_post_tr(2)) and the ZLF boundary
-!     flux (Adz_BC_LAM_flux = 2) are not part of this library: they stop with a message.
-!     Quintic vertical interpolation (F_QV_L) is (FP32 fields).
+!   * ILMC (mono = 2) and the fixer of the BICHQV group (adz_post_tr(2)) are not part of
+!     this library: they stop with a message.  Quintic vertical interpolation (F_QV_L, FP32
+!     fields) and the ZLF boundary treatment (Adz_BC_LAM_flux = 2: adz_BC_LAM_zlf_0_hlt
+!     around the interpolation on Adz_pb, the fixer on the Adz_pb box) are.
 !   * Bermejo_Conde synchronises the device to return each tracer's mass_deficit in
 !     F_bc(n)%mass_deficit, as the reference's host scalar.
 !   * The LAM boundary flux keeps only what Bermejo_Conde reads of it: the masses of
@@ -42,7 +45,7 @@ module adz_gpu
 
    ! ------------------------------------------------------------ types (adz_mem.F90:66-125, tr3d_mod.F90:63-70)
    type :: Adz_pntr_stack
-      type(c_ptr) :: src = c_null_ptr, dst = c_null_ptr
+      type(c_ptr) :: src = c_null_ptr, dst = c_null_ptr, pil = c_null_ptr
    end type Adz_pntr_stack
 
    type :: post
@@ -283,6 +286,39 @@ contains
       deallocate (tr, tr_num, hm)
    end subroutine adz_BC_LAM_Aranami_hlt
 
+   ! adz_BC_LAM_zlf_hlt (adz_BC_LAM_zlf_hlt.F90:18-319) on n device arrays (element (1,1,1)
+   ! of the model layout); F2 is read or written by setups 2 and 4 only.
+   subroutine adz_BC_LAM_zlf_hlt(F1, F2, F_n, F_setup)
+      integer, intent(in) :: F_n, F_setup
+      type(c_ptr), intent(in) :: F1(F_n), F2(F_n)
+      type(adz_zlf_domain) :: dom
+      if (F_n <= 0) return
+      dom = adz_zlf_domain(l_minx, l_maxx, l_miny, l_maxy, l_nk, l_ni, l_nj, pil_w, pil_e, pil_s, pil_n, &
+                           1 - Adz_lminx, Adz_k0t)
+      call adz_check(adz_bc_lam_zlf(int(F_setup, c_int), F1, F2, int(F_n, c_int), dom, int(Adz_field_dtype, c_int), &
+                                    Adz_stream), 'adz_BC_LAM_zlf_hlt')
+   end subroutine adz_BC_LAM_zlf_hlt
+
+   ! adz_BC_LAM_zlf_0_hlt (adz_BC_LAM_zlf_0_hlt.F90:18-102) on Adz_stack(1:F_nptr):
+   ! F_number 0 before the interpolation (setup 1 on src, setup 2 from dst into pil),
+   ! 1 before adz_post_tr (setup 3 on dst), 2 after it (setup 4 from pil into dst).
+   subroutine adz_BC_LAM_zlf_0_hlt(F_nptr, F_number)
+      integer, intent(in) :: F_nptr, F_number
+      type(c_ptr) :: s(F_nptr), d(F_nptr), p(F_nptr)
+      integer :: n
+      do n = 1, F_nptr
+         s(n) = Adz_stack(n)%src; d(n) = Adz_stack(n)%dst; p(n) = Adz_stack(n)%pil
+      end do
+      if (F_number == 0) then
+         call adz_BC_LAM_zlf_hlt(s, s, F_nptr, 1)
+         call adz_BC_LAM_zlf_hlt(d, p, F_nptr, 2)
+      else if (F_number == 1) then
+         call adz_BC_LAM_zlf_hlt(d, d, F_nptr, 3)
+      else if (F_number == 2) then
+         call adz_BC_LAM_zlf_hlt(d, p, F_nptr, 4)
+      end if
+   end subroutine adz_BC_LAM_zlf_0_hlt
+
    ! F_wp = 1: register the Tr_3CWP tracers that use the fixer (their clipping happened in
    ! adz_tricub_hlt); F_wp = 0: core limits, settings from each tracer's mass code,
    ! Bermejo_Conde, and the carried deficits back into Tr_3CWP.
@@ -309,11 +345,10 @@ contains
          return
       end if
       if (n_bc_TOTAL == 0) return
-      if (Adz_BC_LAM_flux == 2) then  ! adz_post_tr.F90:147-151 and adz_BC_LAM_zlf_*
-         write (error_unit, *) 'adz_post_tr: the ZLF boundary flux (Adz_BC_LAM_flux = 2) is not part of this library'
-         error stop 1
-      end if
       i0 = 1 + pil_w; in = l_ni - pil_e; j0 = 1 + pil_s; jn = l_nj - pil_n; k0 = Adz_k0t
+      if (Adz_BC_LAM_flux == 2) then  ! ZLF: the fixer's scope is the Adz_pb box (adz_post_tr.F90:147-151)
+         i0 = Adz_i0b; in = Adz_inb; j0 = Adz_j0b; jn = Adz_jnb; k0 = 1
+      end if
       do num = 1, n_bc_TOTAL
          n = Adz_bc(num)%n
          Adz_bc(num)%p = Adz_stack(n)%src

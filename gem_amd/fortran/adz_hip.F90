@@ -64,6 +64,13 @@ module adz_hip
       type(c_ptr) :: src = c_null_ptr, fo = c_null_ptr, fi = c_null_ptr
    end type adz_flux_tracer
 
+   type, bind(C) :: adz_zlf_domain
+      integer(c_int) :: minx, maxx, miny, maxy, nk
+      integer(c_int) :: ni, nj
+      integer(c_int) :: pil_w, pil_e, pil_s, pil_n
+      integer(c_int) :: ext, k0
+   end type adz_zlf_domain
+
    type, bind(C) :: adz_flux_domain
       integer(c_int) :: minx, maxx, miny, maxy, nk
       integer(c_int) :: ni, nj
@@ -304,6 +311,14 @@ module adz_hip
          import :: c_int, c_ptr
          type(c_ptr), value :: fx
          integer(c_int), intent(out) :: count
+         integer(c_int) :: rc
+      end function
+      function adz_bc_lam_zlf(setup, f1, f2, nf, dom, dtype, stream) bind(C, name="adz_bc_lam_zlf") result(rc)
+         import :: c_int, c_ptr, adz_zlf_domain
+         integer(c_int), value :: setup, nf, dtype
+         type(c_ptr), intent(in) :: f1(*), f2(*)
+         type(adz_zlf_domain), intent(in) :: dom
+         type(c_ptr), value :: stream
          integer(c_int) :: rc
       end function
       function adz_flux_free(fx) bind(C, name="adz_flux_free") result(rc)

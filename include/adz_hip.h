@@ -346,6 +346,27 @@ int adz_flux_create(const adz_flux_tracer* tr, int ntr, const adz_flux_domain* d
  * boundary are interpolated; the others hold zeros.  Capturable (no host synchronisation). */
 int adz_bc_lam_flux(adz_flux* fx, const void* xyz, const void* airm0, const double* mask, double* mass,
                     adz_stream stream);
+/* The ZLF form (Adz_BC_LAM_flux = 2): the element-wise steps of adz_BC_LAM_zlf_hlt
+ * (adz_BC_LAM_zlf_hlt.F90:18-319) on nf DEVICE arrays (HOST tables f1 / f2 of pointers to
+ * element (1,1,1) of model-layout arrays) over the tile of a one-tile LAM:
+ *   1: f1 (a source) = 0 outside the extension (pilot zones minus ext = Grd_maxcfl + 1
+ *      cells) and, inside it, below level nint(k0/2);
+ *   2: f2 (the piloting copy, tracers_B) = f1 (the destination) outside the core;
+ *   3: f1 = 0 outside [5, ni-4] x [5, nj-4];
+ *   4: f1 = f2 outside the core.
+ * The core is 1+pil_w:ni-pil_e x 1+pil_s:nj-pil_n x k0:nk with k0 = Adz_k0t.  The caller
+ * runs them as adz_BC_LAM_zlf_0_hlt does (adz_BC_LAM_zlf_0_hlt.F90:55-95): 1 and 2 before
+ * interpolating on Adz_pb, 3 before the fixer (whose scope is then Adz_i0b:Adz_inb x
+ * Adz_j0b:Adz_jnb x 1:nk, adz_post_tr.F90:147-151), 4 after it. */
+typedef struct adz_zlf_domain {
+  int minx, maxx, miny, maxy, nk; /* array layout                      */
+  int ni, nj;                     /* the tile                          */
+  int pil_w, pil_e, pil_s, pil_n; /* Glb_pil_* of the four LAM edges   */
+  int ext;                        /* Grd_maxcfl + 1                    */
+  int k0;                         /* Adz_k0t                           */
+} adz_zlf_domain;
+int adz_bc_lam_zlf(int setup, void* const* f1, void* const* f2, int nf, const adz_zlf_domain* dom, int dtype,
+                   adz_stream stream);
 /* Number of targets interpolated by the last adz_bc_lam_flux (synchronises; diagnostics).
  * No counterpart in the reference, which interpolates every target. */
 int adz_flux_active(const adz_flux* fx, int* count);

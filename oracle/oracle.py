@@ -510,3 +510,44 @@ class RefQvZyx:
                C.c_void_p(ints.ctypes.data), C.c_void_p(ints.ctypes.data + 4), C.c_void_p(ints.ctypes.data + 8),
                C.c_void_p(ints.ctypes.data + 12))
         return out
+
+
+def bc_lam_zlf(setup, f1, f2, layout, tile, pil, ext, k0):
+    """adz_BC_LAM_zlf_hlt.F90:18-319 for a tile whose four sides are LAM edges, in place on
+    flat arrays of `layout` = (minx, maxx, miny, maxy, nk) (array start); pil = (pil_w,
+    pil_e, pil_s, pil_n); ext = Grd_maxcfl + 1; k0 = Adz_k0t."""
+    minx, maxx, miny, maxy, nk = layout
+    ni, nj = tile
+    pil_w, pil_e, pil_s, pil_n = pil
+    shape = (nk, maxy - miny + 1, maxx - minx + 1)
+    F1 = f1.reshape(shape)
+    F2 = f2.reshape(shape) if f2 is not None else None
+
+    def box(ia, ib, ja, jb, ka, kb):  # i = ia..ib, j = ja..jb, k = ka..kb (1-based, inclusive; empty if b < a)
+        if ib < ia or jb < ja or kb < ka:
+            return (slice(0, 0), slice(0, 0), slice(0, 0))
+        return (slice(ka - 1, kb), slice(ja - miny, jb + 1 - miny), slice(ia - minx, ib + 1 - minx))
+
+    if setup == 1:                                            # :44-95
+        i0e, ine = 1 + pil_w - ext, ni - pil_e + ext
+        j0e, jne = 1 + pil_s - ext, nj - pil_n + ext
+        k0d2 = int(np.floor(k0 / 2.0 + 0.5))                  # nint(k0/2.) for k0 >= 0
+        F1[box(1, i0e - 1, 1, nj, 1, nk)] = 0
+        F1[box(ine + 1, ni, 1, nj, 1, nk)] = 0
+        F1[box(1, ni, 1, j0e - 1, 1, nk)] = 0
+        F1[box(1, ni, jne + 1, nj, 1, nk)] = 0
+        F1[box(i0e, ine, j0e, jne, 1, k0d2 - 1)] = 0
+    elif setup in (2, 4):                                     # :97-151, :212-268
+        src, dst = (F1, F2) if setup == 2 else (F2, F1)
+        i0c, inc, j0c, jnc = 1 + pil_w, ni - pil_e, 1 + pil_s, nj - pil_n
+        for b in (box(1, i0c - 1, 1, nj, 1, nk), box(inc + 1, ni, 1, nj, 1, nk), box(1, ni, 1, j0c - 1, 1, nk),
+                  box(1, ni, jnc + 1, nj, 1, nk), box(i0c, inc, j0c, jnc, 1, k0 - 1)):
+            dst[b] = src[b]
+    elif setup == 3:                                          # :153-210
+        B = 4
+        F1[box(1, B, 1, nj, 1, nk)] = 0
+        F1[box(ni - B + 1, ni, 1, nj, 1, nk)] = 0
+        F1[box(1, ni, 1, B, 1, nk)] = 0
+        F1[box(1, ni, nj - B + 1, nj, 1, nk)] = 0
+    else:
+        raise ValueError("ADZ_BC_LAM_ZLF: SETUP not defined")

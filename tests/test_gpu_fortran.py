@@ -5,7 +5,9 @@ adz_post_tr(0) -> Bermejo_Conde -- and its results are bit-identical to the same
 called through the C ABI from Python: departure positions, interpolated and clipped
 tracers, Adz_post lin/min/max, fixed tracers and the carried mass deficits (the fixer with
 the LAM boundary flux, Adz_BC_LAM_flux = 1).  Both interpolation forms: tricubic, and
-bicubic-horizontal / quintic-vertical (F_QV_L = .true.)."""
+bicubic-horizontal / quintic-vertical (F_QV_L = .true.); and the ZLF boundary treatment
+(Adz_BC_LAM_flux = 2: interpolation on Adz_pb, piloting values saved and restored, the
+fixer on the Adz_pb box)."""
 import os
 import subprocess
 
@@ -20,7 +22,7 @@ DRIVER = os.path.join(ROOT, "gem_amd", "fortran", "adz_f_driver")
 MH = 2
 
 
-@pytest.mark.parametrize("form", ["tricubic", "quintic"])
+@pytest.mark.parametrize("form", ["tricubic", "quintic", "zlf"])
 def test_fortran_step_matches_c_abi(cuda, tmp_path, form):
     if not os.path.exists(DRIVER):
         pytest.fail("gem_amd/fortran/adz_f_driver is not built")
@@ -63,11 +65,13 @@ def test_fortran_step_matches_c_abi(cuda, tmp_path, form):
         mass.tofile(f)
         mono.tofile(f)
     print("[gpu-case] running the Fortran driver", flush=True)
-    r = subprocess.run([DRIVER, str(fin), str(fout)] + (["qv"] if form == "quintic" else []), capture_output=True,
-                       text=True, timeout=120)
+    mode = {"tricubic": [], "quintic": ["qv"], "zlf": ["zlf"]}[form]
+    r = subprocess.run([DRIVER, str(fin), str(fout)] + mode, capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
 
     # the same step through the C ABI
+    if form == "zlf":
+        g["bc_lam_flux"] = 2
     ctx = A.AdzContext(g, L.ADZ_F32, itraj=cfg.niter)
     wd = {k: v.cuda() for k, v in w.items()}
     ctx.adz_inittraj()
@@ -79,12 +83,22 @@ def test_fortran_step_matches_c_abi(cuda, tmp_path, form):
     ent = [dict(src=torch.from_numpy(t).cuda(), dst=torch.zeros(n_mod, device="cuda"),
                 lin=torch.zeros(n_mod, device="cuda"), min=torch.zeros(n_mod, device="cuda"),
                 max=torch.zeros(n_mod, device="cuda")) for t in trc]
+    lay5 = dl + (nk,)
+    zlf = dict(layout=lay5, tile=(cfg.ni, cfg.nj), pil=(g["pil_w"], g["pil_e"], g["pil_s"], g["pil_n"]),
+               ext=cfg.halo, k0=i["k0t"], off=off)
+    pil_c = [torch.zeros(n_mod, device="cuda") for _ in ent]
+    if form == "zlf":  # adz_BC_LAM_zlf_0_hlt(n, 0)
+        A.adz_bc_lam_zlf(1, [e["src"] for e in ent], None, **zlf)
+        A.adz_bc_lam_zlf(2, [e["dst"] for e in ent], pil_c, **zlf)
     for clip in (0, 1):
         grp = [e for e, m in zip(ent, mono) if (m == 1) == bool(clip)]
         post = A.ADZ_POST_STORE | (A.ADZ_POST_CLIP if clip else 0)
         if not grp:
             continue
-        if form == "quintic":
+        if form == "zlf":
+            A.adz_tricub_hlt(grp, ctx.array(A.ADZ_A_PB), ctx.ztab(True), i["i0b"], i["inb"], i["j0b"], i["jnb"], 1,
+                             nk, dl, ext=False, post=post, math=L.ADZ_MATH_REF, src_off=off)
+        elif form == "quintic":
             A.adz_tricub_hlt_qv(ctx, grp, ctx.array(A.ADZ_A_PT), *box[:5], dl, ext=False, post=post, src_off=off)
         else:
             A.adz_tricub_hlt(grp, ctx.array(A.ADZ_A_PT), ctx.ztab(True), *box[:5], box[5], dl, ext=False,
@@ -93,15 +107,23 @@ def test_fortran_step_matches_c_abi(cuda, tmp_path, form):
     bcs = [dict(p=ent[n]["src"], m=ent[n]["dst"], lin=ent[n]["lin"], min=ent[n]["min"], max=ent[n]["max"],
                 mass=int(mass[n])) for n in bcn]
     scope = (1 + g["pil_w"], cfg.ni - g["pil_e"], 1 + g["pil_s"], cfg.nj - g["pil_n"], i["k0t"])
+    if form == "zlf":  # adz_BC_LAM_zlf_0_hlt(n, 1); the fixer on the Adz_pb box from level 1
+        A.adz_bc_lam_zlf(3, [e["dst"] for e in ent], None, **zlf)
+        scope = (i["i0b"], i["inb"], i["j0b"], i["jnb"], 1)
     bc = A.BermejoConde(bcs, dl + (nk,), scope, off, min_max=True)
     deficit = torch.zeros(len(bcn), dtype=torch.float64, device="cuda")
     d_air0, d_mask = airm0.reshape(-1).cuda(), mask.reshape(-1).double().cuda()
-    # the LAM boundary flux (Adz_BC_LAM_flux = 1, the driver's grid) of the fixer's tracers
-    fx = A.BcLamFlux([dict(src=ent[n]["src"]) for n in bcn], dl + (nk,), (cfg.ni, cfg.nj),
-                     (i["i0b"], i["inb"], i["j0b"], i["jnb"]), scope, off, ctx.ztab(True), pos_dtype=L.ADZ_F32)
-    fmass = torch.zeros(2 * len(bcn), dtype=torch.float64, device="cuda")
-    fx(ctx.array(A.ADZ_A_PB), d_air0, d_mask, fmass)
-    bc(airm1.reshape(-1).cuda(), d_air0, d_mask, deficit, pkps=pkps.reshape(-1).cuda(), flux=fmass)
+    fx = None
+    if form == "zlf":
+        bc(airm1.reshape(-1).cuda(), d_air0, d_mask, deficit, pkps=pkps.reshape(-1).cuda())
+        A.adz_bc_lam_zlf(4, [e["dst"] for e in ent], pil_c, **zlf)  # adz_BC_LAM_zlf_0_hlt(n, 2)
+    else:
+        # the LAM boundary flux (Adz_BC_LAM_flux = 1, the driver's grid) of the fixer's tracers
+        fx = A.BcLamFlux([dict(src=ent[n]["src"]) for n in bcn], dl + (nk,), (cfg.ni, cfg.nj),
+                         (i["i0b"], i["inb"], i["j0b"], i["jnb"]), scope, off, ctx.ztab(True), pos_dtype=L.ADZ_F32)
+        fmass = torch.zeros(2 * len(bcn), dtype=torch.float64, device="cuda")
+        fx(ctx.array(A.ADZ_A_PB), d_air0, d_mask, fmass)
+        bc(airm1.reshape(-1).cuda(), d_air0, d_mask, deficit, pkps=pkps.reshape(-1).cuda(), flux=fmass)
     torch.cuda.synchronize()
 
     npt = 3 * (box[1] - box[0] + 1) * (box[3] - box[2] + 1) * (nk - box[4] + 1)
@@ -128,7 +150,8 @@ def test_fortran_step_matches_c_abi(cuda, tmp_path, form):
     exp[bcn] = deficit.cpu().numpy()
     assert np.array_equal(fdef.view(np.uint64), exp.view(np.uint64))
     assert np.any(exp != 0.0)
-    assert np.all(fmass.cpu().numpy() != 0.0)
-    fx.free()
+    if fx is not None:
+        assert np.all(fmass.cpu().numpy() != 0.0)
+        fx.free()
     bc.free()
     ctx.free()
