@@ -225,18 +225,35 @@ __global__ __launch_bounds__(kBcBlock) void k_bc_pass(BcArgs A) {
   }
 }
 
-// Adds the block partials of every tracer in index order and runs the scalar steps that
-// follow each global sum (one lane per tracer).
+// One block per tracer: lane l merges the block partials l, l+256, ... in order, a fixed
+// tree merges the lanes, and lane 0 runs the scalar steps that follow each global sum.
+constexpr int kFinBlock = 256;
 template <int PASS>
-__global__ void k_bc_fin(BcArgs A) {
-  int n = blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= A.ntr) return;
+__global__ __launch_bounds__(kFinBlock) void k_bc_fin(BcArgs A) {
+  const int n = blockIdx.x, l = threadIdx.x;
   constexpr int NS = PASS == P_PM ? 4 : 1;
   DD s[4] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
   const double* pp = A.part + (long)n * A.nblk * 8;
-  for (int b = 0; b < A.nblk; b++)
+  for (int b = l; b < A.nblk; b += kFinBlock)
 #pragma unroll
     for (int q = 0; q < NS; q++) s[q] = dd_merge(s[q], DD{pp[8 * b + 2 * q], pp[8 * b + 2 * q + 1]});
+  __shared__ double sh[2 * NS][kFinBlock];
+#pragma unroll
+  for (int q = 0; q < NS; q++) { sh[2 * q][l] = s[q].hi; sh[2 * q + 1][l] = s[q].lo; }
+  __syncthreads();
+  for (int w = kFinBlock / 2; w > 0; w >>= 1) {
+    if (l < w) {
+#pragma unroll
+      for (int q = 0; q < NS; q++) {
+        DD c = dd_merge(DD{sh[2 * q][l], sh[2 * q + 1][l]}, DD{sh[2 * q][l + w], sh[2 * q + 1][l + w]});
+        sh[2 * q][l] = c.hi; sh[2 * q + 1][l] = c.lo;
+      }
+    }
+    __syncthreads();
+  }
+  if (l != 0) return;
+#pragma unroll
+  for (int q = 0; q < NS; q++) s[q] = DD{sh[2 * q][0], sh[2 * q + 1][0]};
   double sum0 = s[0].hi + s[0].lo + 0.0;
   double* sc = A.sc + n * kSc;
   if (PASS == P_PM) {
@@ -349,7 +366,7 @@ int bc_apply(adz_bc* bc, const void* airm1, const void* airm0, const void* pkps,
   if (bc->has_w3 && !pkps) return fail(ADZ_EINVAL, "adz_bermejo_conde: weight 3 needs pkps");
   hipStream_t st = (hipStream_t)stream;
   dim3 g((unsigned)((long)a.ntr * a.nblk)), b(kBcBlock);
-  dim3 gf((unsigned)((a.ntr + 63) / 64)), bf(64);
+  dim3 gf((unsigned)a.ntr), bf(kFinBlock);
 #define ADZ_BC_PASSES(T)                                                       \
   hipLaunchKernelGGL((k_bc_pass<T, P_PM>), g, b, 0, st, a);                   \
   hipLaunchKernelGGL((k_bc_fin<P_PM>), gf, bf, 0, st, a);                     \

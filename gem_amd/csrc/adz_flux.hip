@@ -12,7 +12,7 @@
 // The GPU form works on those rings only:
 //   1. k_flux_mark: one lane per target: index step, class (inactive, FLUX_out, FLUX_in),
 //      count per block;
-//   2. k_flux_scan: exclusive scan of the block counts (one block);
+//   2. k_flux_scan, k_flux_scan_chunks: exclusive scan of the block counts;
 //   3. k_flux_emit: the active targets, in target order, into a list;
 //   4. k_flux_eval: per (group of tracers, slice of the list): the masked tricubic
 //      interpolation in the reference's arithmetic, optional store into fo / fi,
@@ -54,7 +54,8 @@ struct FluxArgs {
   int ntr, nslices, store;
   const FluxTr* tr;
   unsigned char* cls;            // per target: bit0 active, bit1 in the core, bit2 bad position
-  int* cnt;                      // per mark block: count, then exclusive offset
+  int* cnt;                      // per mark block: count, then exclusive offset within its chunk
+  int* chunk;                    // per 1024 mark blocks: total, then exclusive offset
   int* total;
   unsigned* list;                // active targets: index | core << 30 | bad << 31
   double* part;                  // [ntr][nslices][4]: FLUX_out hi lo, FLUX_in hi lo
@@ -108,29 +109,44 @@ __global__ __launch_bounds__(kFxBlock) void k_flux_mark(FluxArgs A) {
   if (threadIdx.x == 0) A.cnt[blockIdx.x] = n;
 }
 
-// Exclusive scan of n block counts in place; one block of 1024 lanes, each owning a
-// contiguous run of counts.
-__global__ __launch_bounds__(1024) void k_flux_scan(int* cnt, int n, int* total) {
-  __shared__ int s[1024];
-  const int t = threadIdx.x, per = (n + 1023) / 1024;
-  const int b0 = min(n, t * per), b1 = min(n, b0 + per);
-  int sum = 0;
-  for (int b = b0; b < b1; b++) sum += cnt[b];
-  s[t] = sum;
+// Exclusive scan of the n block counts in two levels, with coalesced loads: each block of
+// 1024 lanes scans 1024 consecutive counts in place and writes its total to chunk[b];
+// k_flux_scan_chunks then scans the chunk totals (one block; chunk[nchunks] = the total).
+constexpr int kScan = 1024;
+__device__ __forceinline__ int block_scan_excl(int v, int* s) {  // s: kScan ints of LDS
+  const int t = threadIdx.x;
+  s[t] = v;
   __syncthreads();
-  for (int o = 1; o < 1024; o <<= 1) {
-    const int v = t >= o ? s[t - o] : 0;
+  for (int o = 1; o < kScan; o <<= 1) {
+    const int u = t >= o ? s[t - o] : 0;
     __syncthreads();
-    s[t] += v;
+    s[t] += u;
     __syncthreads();
   }
-  int run = s[t] - sum;
-  for (int b = b0; b < b1; b++) {
-    const int c = cnt[b];
-    cnt[b] = run;
-    run += c;
+  return s[t] - v;
+}
+
+__global__ __launch_bounds__(kScan) void k_flux_scan(int* cnt, int n, int* chunk) {
+  __shared__ int s[kScan];
+  const long b = (long)blockIdx.x * kScan + threadIdx.x;
+  const int v = b < n ? cnt[b] : 0;
+  const int e = block_scan_excl(v, s);
+  if (b < n) cnt[b] = e;
+  if (threadIdx.x == kScan - 1) chunk[blockIdx.x] = e + v;
+}
+
+__global__ __launch_bounds__(kScan) void k_flux_scan_chunks(int* chunk, int nchunks, int* total) {
+  __shared__ int s[kScan];
+  int run = 0;
+  for (int c0 = 0; c0 < nchunks; c0 += kScan) {  // nchunks <= 2^30 / 256 / 1024: a few passes at most
+    const int c = c0 + (int)threadIdx.x;
+    const int v = c < nchunks ? chunk[c] : 0;
+    const int e = block_scan_excl(v, s);
+    if (c < nchunks) chunk[c] = run + e;
+    run += s[kScan - 1];
+    __syncthreads();
   }
-  if (t == 1023) *total = s[1023];
+  if (threadIdx.x == 0) *total = run;
 }
 
 __global__ __launch_bounds__(kFxBlock) void k_flux_emit(FluxArgs A) {
@@ -143,7 +159,7 @@ __global__ __launch_bounds__(kFxBlock) void k_flux_emit(FluxArgs A) {
   const int below = __popcll(m & ((1ull << lane) - 1ull));
   if (lane == 0) wsum[w] = __popcll(m);
   __syncthreads();
-  int base = A.cnt[blockIdx.x];
+  int base = A.cnt[blockIdx.x] + A.chunk[blockIdx.x / kScan];
   for (int q = 0; q < w; q++) base += wsum[q];
   if (act) A.list[base + below] = (unsigned)p | ((unsigned)(c & 6) << 29);
 }
@@ -344,8 +360,10 @@ int adz_flux_create(const adz_flux_tracer* tr, int ntr, const adz_flux_domain* d
   std::vector<FluxTr> h(ntr);
   for (int n = 0; n < ntr; n++) h[n] = FluxTr{tr[n].src, tr[n].fo, tr[n].fi};
   auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  const int nchunk = (nmark + kScan - 1) / kScan;
   const size_t tb = up(sizeof(FluxTr) * ntr), pb = up(sizeof(double) * 4 * ntr * a.nslices),
-               cb = up(sizeof(int) * (nmark + 1)), lb = up(sizeof(unsigned) * a.npts), kb = up((size_t)a.npts);
+               cb = up(sizeof(int) * (nmark + 1 + nchunk)), lb = up(sizeof(unsigned) * a.npts),
+               kb = up((size_t)a.npts);
   char* mem = nullptr;
   if (hipMalloc((void**)&mem, tb + pb + cb + lb + kb) != hipSuccess) return fail(ADZ_ENOMEM, "adz_flux_create: hipMalloc");
   if (hipMemcpy(mem, h.data(), sizeof(FluxTr) * ntr, hipMemcpyHostToDevice) != hipSuccess) {
@@ -356,6 +374,7 @@ int adz_flux_create(const adz_flux_tracer* tr, int ntr, const adz_flux_domain* d
   a.part = (double*)(mem + tb);
   a.cnt = (int*)(mem + tb + pb);
   a.total = a.cnt + nmark;
+  a.chunk = a.total + 1;
   a.list = (unsigned*)(mem + tb + pb + cb);
   a.cls = (unsigned char*)(mem + tb + pb + cb + lb);
   adz_flux* f = new adz_flux;
@@ -373,7 +392,9 @@ int adz_bc_lam_flux(adz_flux* fx, const void* xyz, const void* airm0, const doub
   const dim3 gm((unsigned)fx->nmark), b(kFxBlock);
   if (fx->pdtype == ADZ_F32) hipLaunchKernelGGL((k_flux_mark<float>), gm, b, 0, st, a);
   else hipLaunchKernelGGL((k_flux_mark<double>), gm, b, 0, st, a);
-  hipLaunchKernelGGL(k_flux_scan, dim3(1), dim3(1024), 0, st, a.cnt, fx->nmark, a.total);
+  const int nchunk = (fx->nmark + kScan - 1) / kScan;
+  hipLaunchKernelGGL(k_flux_scan, dim3((unsigned)nchunk), dim3(kScan), 0, st, a.cnt, fx->nmark, a.chunk);
+  hipLaunchKernelGGL(k_flux_scan_chunks, dim3(1), dim3(kScan), 0, st, a.chunk, nchunk, a.total);
   hipLaunchKernelGGL(k_flux_emit, gm, b, 0, st, a);
   if (a.store) {
     const dim3 gz((unsigned)fx->nmark, (unsigned)a.ntr);
