@@ -165,3 +165,27 @@ def test_flux_graph_replays(W):
         fx(W["pb"], W["airm0"], W["mask"], eager)  # eager work between the replays
         torch.cuda.synchronize()
     fx.free()
+
+
+def test_flux_with_no_active_target(W):
+    """A core that covers the whole tile: no target outside it, no stencil reaching outside it.
+    The masses are exactly +0 and the stored arrays stay zero on the target box."""
+    torch, A = W["torch"], W["A"]
+    core = (1, W["ni"], 1, W["nj"], 1)
+    dev = [dict(src=torch.from_numpy(t.astype(np.float32)).cuda(), fo=torch.full((t.size,), 7.0, device="cuda"),
+                fi=torch.full((t.size,), 7.0, device="cuda")) for t in W["tr"]]
+    fx = A.BcLamFlux(dev, W["layout"], (W["ni"], W["nj"]), W["bbox"], core, W["off"], W["ctx"].ztab(True),
+                     pos_dtype=W["pdt"])
+    mass = torch.full((2 * len(dev),), -1.0, dtype=torch.float64, device="cuda")
+    fx(W["pb"], W["airm0"], W["mask"], mass)
+    torch.cuda.synchronize()
+    assert fx.active() == 0
+    assert np.array_equal(_bits(mass.cpu().numpy()), _bits(np.zeros(2 * len(dev))))
+    i0b, inb, j0b, jnb = W["bbox"]
+    shape = (W["nk"], W["ny"], W["nx"])
+    sl = (slice(None), slice(j0b - 1 + MH, jnb + MH), slice(i0b - 1 + MH, inb + MH))
+    for d in dev:
+        assert np.all(d["fo"].cpu().numpy().reshape(shape)[sl] == 0.0)
+        assert np.all(d["fi"].cpu().numpy().reshape(shape)[sl] == 0.0)
+        assert np.all(d["fo"].cpu().numpy().reshape(shape)[:, :MH, :] == 7.0)  # outside the box: untouched
+    fx.free()

@@ -80,3 +80,23 @@ def test_qv_bit_exact(W, ext, post):
             assert np.array_equal(got(stack[f], "lin").view(np.uint32), el.view(np.uint32)), f
             assert np.array_equal(got(stack[f], "min").view(np.uint32), mi[f].view(np.uint32)), f
             assert np.array_equal(got(stack[f], "max").view(np.uint32), ma[f].view(np.uint32)), f
+
+
+def test_qv_positions_outside_the_source_give_nan(W):
+    """A stencil that would leave the source arrays yields NaN (the reference reads out of
+    bounds there); the other targets are unaffected."""
+    torch, A, cfg, ctx = W["torch"], W["A"], W["cfg"], W["ctx"]
+    i = ctx.info
+    dl = (1 - MH, cfg.ni + MH, 1 - MH, cfg.nj + MH)
+    n_dst = (dl[1] - dl[0] + 1) * (dl[3] - dl[2] + 1) * cfg.nk
+    src = torch.ones(n_dst, device="cuda")
+    box = (i["i0"], i["i0"] + 3, i["j0"], i["j0"], cfg.nk)  # 4 targets on the top level
+    pos = torch.tensor([[3.5, 5.5, 20.5], [-50.0, 5.5, 20.5], [3.5, 1e12, 20.5], [float("nan"), 5.5, 20.5]],
+                       dtype=ctx.array(A.ADZ_A_PT).dtype, device="cuda").reshape(-1)
+    stack = [dict(src=src, dst=torch.zeros(n_dst, device="cuda"))]
+    A.adz_tricub_hlt_qv(ctx, stack, pos, *box, dl, ext=False, post=0)
+    torch.cuda.synchronize()
+    shape = (cfg.nk, dl[3] - dl[2] + 1, dl[1] - dl[0] + 1)
+    got = stack[0]["dst"].cpu().numpy().reshape(shape)[cfg.nk - 1, box[2] - dl[2], box[0] - dl[0]:box[1] - dl[0] + 1]
+    assert got[0] == 1.0
+    assert np.all(np.isnan(got[1:]))
