@@ -388,6 +388,67 @@ class BermejoConde:
             pass
 
 
+# ---------------------------------------------------------------------- ILMC (mono = 2)
+L.extra_sigs({
+    "adz_ilmc_create": ("i", ["vp", "i", "vp", "vp"]),
+    "adz_ilmc_lam": ("i", ["vp", "vp", "vp", "vp"]),
+    "adz_ilmc_free": ("i", ["vp"]),
+})
+
+
+class AdzIlmcTracer(C.Structure):
+    _fields_ = [("f", C.c_void_p), ("mn", C.c_void_p), ("mx", C.c_void_p)]
+
+
+class AdzIlmcDomain(C.Structure):
+    _fields_ = [(n, C.c_int) for n in ("minx", "maxx", "miny", "maxy", "nk", "i0", "in_", "j0", "jn", "k0",
+                                       "sweep_max", "min_max", "autobar")]
+
+
+class IlmcLam:
+    """ILMC_LAM (ILMC_LAM.F90:18-273, ilmc_lam_loop.inc:16-139): ILMC shape preservation of
+    the tracers whose mono code is 2 (adz_post_tr.F90:104), on device arrays.
+
+    tracers: list of dicts with FP32 tensors `f` (the interpolated tracer, corrected in
+    place), `min`, `max` (Adz_post min/max), model layout with element (1,1,1) at offset
+    `off`.  box = (i0, in, j0, jn, k0): Adz_i0..Adz_in x Adz_j0..Adz_jn x Adz_k0t..nk.
+    sweep_max, min_max: Adz_ILMC_sweep_max, Adz_ILMC_min_max_L; autobar: Schm_autobar_L."""
+
+    def __init__(self, tracers, layout, box, off, sweep_max=2, min_max=True, autobar=False):
+        minx, maxx, miny, maxy, nk = layout
+        i0, in_, j0, jn, k0 = box
+        arr = (AdzIlmcTracer * len(tracers))()
+        for n, t in enumerate(tracers):
+            _check_types(torch.float32, f=t["f"], min=t["min"], max=t["max"])
+            arr[n].f, arr[n].mn, arr[n].mx = _ptr_at(t["f"], off), _ptr_at(t["min"], off), _ptr_at(t["max"], off)
+        dom = AdzIlmcDomain(minx, maxx, miny, maxy, nk, i0, in_, j0, jn, k0, sweep_max, 1 if min_max else 0,
+                            1 if autobar else 0)
+        h = C.c_void_p()
+        L.check(L.lib().adz_ilmc_create(arr, len(tracers), C.byref(dom), C.byref(h)), "adz_ilmc_create")
+        self.h, self.off, self.ntr = h, off, len(tracers)
+
+    def __call__(self, air_m, counts=None):
+        """air_m: FP32 air mass at time M (airm0).  counts: optional int64 tensor [3*ntr]
+        receiving per tracer the sweeps that moved mass, min resets and max resets."""
+        _check_types(torch.float32, air_m=air_m)
+        _check_types(torch.int64, counts=counts)
+        if counts is not None and counts.numel() < 3 * self.ntr:
+            raise ValueError("counts needs 3 values per tracer")
+        L.check(L.lib().adz_ilmc_lam(self.h, _ptr_at(air_m, self.off), None if counts is None else counts.data_ptr(),
+                                     _stream()), "adz_ilmc_lam")
+
+    def free(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            L.lib().adz_ilmc_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
 # ---------------------------------------------------------------------- LAM boundary flux
 class AdzFluxTracer(C.Structure):
     _fields_ = [("src", C.c_void_p), ("fo", C.c_void_p), ("fi", C.c_void_p)]

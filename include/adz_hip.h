@@ -372,6 +372,42 @@ int adz_bc_lam_zlf(int setup, void* const* f1, void* const* f2, int nf, const ad
 int adz_flux_active(const adz_flux* fx, int* count);
 int adz_flux_free(adz_flux* fx);
 
+/* ------------------------------------------------------------------------------------
+ * Level 4c: ILMC shape preservation (mono = 2): ILMC_LAM (ILMC_LAM.F90:18-273) with its
+ * sweep loop (ilmc_lam_loop.inc:16-139), called from adz_post_tr.F90:104 for each tracer
+ * whose mono code is 2.  For a tile that touches the four LAM edges: the domain is
+ * i0..in x j0..jn x k0..nk (Adz_i0..Adz_in x Adz_j0..Adz_jn x Adz_k0t..nk,
+ * ILMC_LAM.F90:81-86).  Arrays are FP32 (`real`) in the model layout; pointers address
+ * element (1,1,1). */
+typedef struct adz_ilmc_tracer {
+  void* f;        /* Adz_stack(n)%dst after interpolation, corrected in place */
+  const void* mn; /* Adz_post(n)%min                                          */
+  const void* mx; /* Adz_post(n)%max                                          */
+} adz_ilmc_tracer;
+
+typedef struct adz_ilmc_domain {
+  int minx, maxx, miny, maxy, nk; /* array layout (l_minx:l_maxx, l_miny:l_maxy, nk) */
+  int i0, in, j0, jn, k0;         /* the domain                                     */
+  int sweep_max;                  /* Adz_ILMC_sweep_max, 1..4 (reference default 2) */
+  int min_max;                    /* Adz_ILMC_min_max_L (reference default true)    */
+  int autobar;                    /* Schm_autobar_L: sweeps stay on one level       */
+} adz_ilmc_domain;
+
+typedef struct adz_ilmc adz_ilmc;
+/* Register ntr tracers and allocate the visiting lists once (8 bytes per cell of the
+ * domain per tracer, the worst case of every cell out of bounds).  Replaces the per-cell
+ * sweep lists the reference allocates on its first call (ILMC_LAM.F90:138-220); the sweep
+ * cells are enumerated in the kernel instead. */
+int adz_ilmc_create(const adz_ilmc_tracer* tr, int ntr, const adz_ilmc_domain* dom, adz_ilmc** out);
+/* Apply ILMC to every registered tracer; air_m is the DEVICE FP32 air mass at time M
+ * (airm0).  The result equals the reference's sequential (k, j, i) visit bit for bit.
+ * counts (DEVICE int64 [3*ntr] or NULL) receives per tracer: sweeps that moved mass
+ * (reset(:,5)), min resets and max resets (reset(:,3:4)).  Tracer arrays must not
+ * overlap.  No host synchronisation: capturable (one launch per hyperplane of the
+ * domain, about ni + (2S+1) nj + (2S+1)^2 nk launches). */
+int adz_ilmc_lam(adz_ilmc* il, const void* air_m, long long* counts, adz_stream stream);
+int adz_ilmc_free(adz_ilmc* il);
+
 #ifdef __cplusplus
 }
 #endif

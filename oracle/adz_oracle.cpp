@@ -282,3 +282,21 @@ extern "C" void ora_qv_zyx(float* out4, const float* src, const double* w, long 
   qv_zyx(out4, out4 + 1, out4 + 2, out4 + 3, src, w, w + 4, w + 8, w + 14, w + 16, w + 18, ni, ninj, zq != 0,
          zc != 0);
 }
+
+// -------------------------------- ILMC (mono = 2) -----------------------------------------
+#include "oracle_ilmc.h"
+
+// dom = minx maxx miny maxy nk i0 in j0 jn k0 sweep_max min_max autobar
+extern "C" void ora_ilmc(float* f, const float* mn, const float* mx, const float* air, const int* dom,
+                         long long* counts) {
+  IlmcDom d{dom[0], dom[1], dom[2], dom[3], dom[4], dom[5], dom[6], dom[7], dom[8], dom[9], dom[10], dom[11], dom[12]};
+  ilmc_lam(f, mn, mx, air, d, counts);
+}
+// the cells of one sweep (array offsets, reference order); returns their number
+extern "C" long ora_ilmc_shell(const int* dom, int i, int j, int k, int s, long* out) {
+  IlmcDom d{dom[0], dom[1], dom[2], dom[3], dom[4], dom[5], dom[6], dom[7], dom[8], dom[9], dom[10], dom[11], dom[12]};
+  std::vector<long> sh;
+  ilmc_shell(d, i, j, k, s, sh);
+  for (size_t q = 0; q < sh.size(); q++) out[q] = sh[q];
+  return (long)sh.size();
+}

@@ -18,6 +18,7 @@ LIB = os.path.join(HERE, "liboracle.so")
 REF_AVX2 = os.path.join(HERE, "_ref", "libtricublin_ref_avx2.so")
 REF_SCALAR = os.path.join(HERE, "_ref", "libtricublin_ref_scalar.so")
 REF_QVZYX = os.path.join(HERE, "_ref", "libqvzyx_ref.so")
+REF_ILMC = os.path.join(HERE, "_ref", "libilmc_ref.so")
 
 ORD_AVX2, ORD_SCALAR = 0, 1
 _lib = None
@@ -64,6 +65,9 @@ def lib():
         L.ora_qv_coef.restype = l
         L.ora_qv_coef.argtypes = [vp, vp, vp, vp, vp]
         L.ora_qv_zyx.argtypes = [vp, vp, vp, l, l, i, i]
+        L.ora_ilmc.argtypes = [vp, vp, vp, vp, vp, vp]
+        L.ora_ilmc_shell.restype = l
+        L.ora_ilmc_shell.argtypes = [vp, i, i, i, i, vp]
         _lib = L
     return _lib
 
@@ -551,3 +555,45 @@ def bc_lam_zlf(setup, f1, f2, layout, tile, pil, ext, k0):
         F1[box(1, ni, nj - B + 1, nj, 1, nk)] = 0
     else:
         raise ValueError("ADZ_BC_LAM_ZLF: SETUP not defined")
+
+
+def _ilmc_dom(layout, box, sweep_max, min_max, autobar):
+    minx, maxx, miny, maxy, nk = layout
+    i0, in_, j0, jn, k0 = box
+    return np.array([minx, maxx, miny, maxy, nk, i0, in_, j0, jn, k0, sweep_max, int(min_max), int(autobar)], np.int32)
+
+
+def ilmc(f, mn, mx, air, layout, box, sweep_max=2, min_max=True, autobar=False):
+    """ILMC_LAM (ILMC_LAM.F90:18-273, ilmc_lam_loop.inc:16-139) in place on the FP32 flat
+    array f of `layout` = (minx, maxx, miny, maxy, nk) over `box` = (i0, in, j0, jn, k0).
+    Returns the counts (sweeps that moved mass, min resets, max resets)."""
+    for a in (f, mn, mx, air):
+        assert a.dtype == np.float32 and a.flags.c_contiguous
+    counts = np.zeros(3, np.int64)
+    lib().ora_ilmc(_p(f), _p(mn), _p(mx), _p(air), _p(_ilmc_dom(layout, box, sweep_max, min_max, autobar)),
+                   _p(counts))
+    return counts
+
+
+def ilmc_shell(layout, box, i, j, k, s, autobar=False):
+    """Array offsets of the cells of sweep s around (i,j,k), in the oracle's visiting order."""
+    out = np.zeros((2 * s + 1) ** 3, np.int64)
+    n = lib().ora_ilmc_shell(_p(_ilmc_dom(layout, box, s, 0, autobar)), i, j, k, s, _p(out))
+    return out[:n].copy()
+
+
+class RefIlmc:
+    """The reference's ILMC sweep loop (ilmc_lam_loop.inc) compiled from its own source
+    inside oracle/ilmc_ref_host.F90 (oracle/_ref).  No min/max reset."""
+
+    def __init__(self):
+        if not os.path.exists(REF_ILMC):
+            raise FileNotFoundError(REF_ILMC)
+        self.L = C.CDLL(REF_ILMC)
+        self.f = self.L.ilmc_ref_loop
+        self.f.argtypes = [C.c_void_p] * 6
+
+    def __call__(self, f, mn, mx, air, layout, box, sweep_max=2, autobar=False):
+        moved = np.zeros(1, np.int64)
+        self.f(_p(f), _p(mn), _p(mx), _p(air), _p(_ilmc_dom(layout, box, sweep_max, 0, autobar)), _p(moved))
+        return int(moved[0])
