@@ -47,6 +47,7 @@ def parse():
     p.add_argument("--err-fields", type=int, default=0, help="fields checked against FP64 (0 = all)")
     p.add_argument("--no-lam-flux", action="store_true",
                    help="c5: leave out the LAM boundary flux of the fixer (Adz_BC_LAM_flux = 0)")
+    p.add_argument("--no-ilmc", action="store_true", help="skip the ILMC (mono = 2) timing beside the step")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--cpu-seconds", type=float, default=12.0)
     p.add_argument("--profile-kernel", action="store_true",
@@ -282,6 +283,64 @@ def main():
         f"interpolation {t_interp:.2f} ms (tracer-group kernel {t_plan0:.2f} ms), Bermejo-Conde {t_bc:.2f} ms, "
         f"LAM flux {t_flux:.2f} ms ({flux_active} of {npts_b} targets interpolated)")
 
+    # ILMC (mono = 2) on one tracer group's cubic results, beside the step (not part of it:
+    # the workload's tracers are fixed by Bermejo-Conde).  Two inputs: the group interpolated
+    # with the reference's arithmetic (what ILMC sees in GEM), and the FP32 form of the plans
+    # (whose constant 1e-12 background lands a few ulp outside [min, max]).  The fields are
+    # restored before each timed call, so every call starts from the same violations.
+    ilmc = None
+    ilmc_cpu_in = ilmc_gpu_out = None
+    Nt_il = npts(tbox)
+    if not args.no_ilmc and groups:
+        ids_l = groups[-1][0]
+        stk = [dict(src=src[q], dst=dst[q], lin=post[k]["lin"], min=post[k]["min"], max=post[k]["max"])
+               for k, q in enumerate(ids_l)]
+        trs = [dict(f=dst[q], min=post[k]["min"], max=post[k]["max"]) for k, q in enumerate(ids_l)]
+        il = A.IlmcLam(trs, layout5, tbox[:5], mod_off, sweep_max=2, min_max=True)
+        counts = torch.zeros(3 * len(ids_l), dtype=torch.int64, device=dev)
+        gil = torch.cuda.CUDAGraph()  # the call as one graph (its launches recorded once)
+        sg = torch.cuda.Stream()
+        sg.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(sg):
+            with torch.cuda.graph(gil, stream=sg):
+                il(airm0, counts)
+        torch.cuda.current_stream().wait_stream(sg)
+        ilmc = {"tracers": len(ids_l), "cells_per_tracer": Nt_il, "sweep_max": 2,
+                "hyperplane_launches": (tbox[1] - tbox[0]) + 5 * (tbox[3] - tbox[2]) + 25 * (cfg.nk - tbox[4]) + 1}
+        for form in ("ref_math", "f32_math"):
+            if form == "ref_math":
+                A.adz_tricub_hlt(stk, pt, ctx.ztab(True), *tbox[:5], tbox[5], mod_layout, ext=False,
+                                 post=A.ADZ_POST_STORE, math=L.ADZ_MATH_REF, src_off=mod_off)
+            else:
+                groups[-1][1].execute()
+            torch.cuda.synchronize()
+            keep = [dst[q].clone() for q in ids_l]
+            viol = sum(int(((dst[q] < post[k]["min"]) | (dst[q] > post[k]["max"])).sum())
+                       for k, q in enumerate(ids_l))
+            if form == "ref_math" and not args.no_cpu:  # one tracer's input for the CPU baseline
+                ilmc_cpu_in = [dst[ids_l[0]].cpu().numpy(), post[0]["min"].cpu().numpy(),
+                               post[0]["max"].cpu().numpy(), airm0.cpu().numpy()]
+            t_il, t_ilg = [], []
+            for rep in range(2):
+                for q, kq in zip(ids_l, keep):
+                    dst[q].copy_(kq)
+                t_ilg.append(timed(gil.replay, 1))
+            for q, kq in zip(ids_l, keep):
+                dst[q].copy_(kq)
+            t_il.append(timed(lambda: il(airm0, counts), 1))
+            cnt = counts.reshape(-1, 3).sum(0).tolist()
+            if form == "ref_math" and not args.no_cpu:
+                ilmc_gpu_out = dst[ids_l[0]].cpu().numpy()
+            del keep
+            ilmc[form] = {"ms": round(min(t_ilg), 3), "ms_eager": round(min(t_il), 3), "cells_out_of_bounds": viol,
+                          "sweeps_moving_mass": cnt[0], "min_resets": cnt[1], "max_resets": cnt[2]}
+            log(f"ILMC beside the step ({form}): {min(t_ilg):.2f} ms in a graph ({min(t_il):.2f} eager) for "
+                f"{len(ids_l)} tracers, {viol} cells out of bounds, {cnt[0]} sweeps moved mass, "
+                f"{ilmc['hyperplane_launches']} hyperplane launches")
+        del gil
+        il.free()
+        torch.cuda.empty_cache()
+
     # algorithmic bytes of the tracer-group kernel (one launch): sources + targets + lin/min/max
     # (3 extra FP32 outputs) + FP64 positions once
     Nt = npts(tbox)
@@ -347,6 +406,11 @@ def main():
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(np, torch, cfg, g, src, pt, tbox, mod_shape, nf, args.cpu_seconds)
+        if ilmc_cpu_in is not None:
+            ilmc["ref_math"].update(ilmc_cpu_baseline(np, ilmc_cpu_in, ilmc_gpu_out, layout5, tbox[:5]))
+            log(f"ILMC CPU baseline (the oracle's sequential loop, 1 thread, one tracer): "
+                f"{ilmc['ref_math']['cpu_ms_per_tracer']:.0f} ms; GPU result bit-exact: "
+                f"{ilmc['ref_math']['gpu_bit_exact_vs_cpu']}")
 
     value = pts_total / (ms_step * 1e-3) / 1e9
     out = {
@@ -376,6 +440,7 @@ def main():
                       "bermejo_conde": round(t_bc, 3), "lam_flux": round(t_flux, 3),
                       "eager_step": round(eager_ms, 3)},
         "lam_flux_targets_interpolated": flux_active,
+        "ilmc_beside_step": ilmc,
         "interp_gpoints_per_s": round(interp_gpts, 3),
         "interp_alg_GBps": round(alg_interp / (t_interp * 1e-3) / 1e9, 1),
         "fp64_plain_gpoints_per_s": round(plain_gpts, 3),
@@ -568,6 +633,20 @@ def measured_traffic(config, tiles):
     if m.get("tiles") != tiles:
         return None
     return m.get("hbm_bytes_per_launch")
+
+
+def ilmc_cpu_baseline(np, inp, gpu_out, layout, box):
+    """ILMC of one tracer by the oracle's restatement of the reference's sequential loop
+    (ilmc_lam_loop.inc), one thread as in the reference, on the same input as the GPU; also
+    checks the GPU result against it bit for bit at full size."""
+    from oracle import oracle as O
+
+    f, mn, mx, air = (np.ascontiguousarray(a) for a in inp)
+    t0 = time.time()
+    O.ilmc(f, mn, mx, air, layout, box, sweep_max=2, min_max=True)
+    ms = (time.time() - t0) * 1e3
+    return {"cpu_ms_per_tracer": round(ms, 1), "cpu_kind": "port", "cpu_cores": 1,
+            "gpu_bit_exact_vs_cpu": bool(np.array_equal(f.view(np.uint32), gpu_out.view(np.uint32)))}
 
 
 def cpu_baseline(np, torch, cfg, g, src, pt, tbox, mod_shape, nf, seconds):

@@ -49,8 +49,8 @@ struct IlmcArgs {
   int a, b;                     // hyperplane coefficients
   int nplanes;
   const float* air;
-  int* cnt;        // [nplanes] cells per hyperplane, then fill cursors
-  int* off;        // [nplanes + 1] bucket starts
+  unsigned long long* cnt;  // [nplanes] cells per hyperplane, then fill cursors
+  long long* off;           // [nplanes + 1] bucket starts
   uint64_t* list;  // cells by hyperplane: i | j << 16 | k << 32 | tracer << 44 (domain offsets)
   long long* counts;
 };
@@ -58,7 +58,7 @@ struct IlmcArgs {
 __device__ __forceinline__ float il_max(float a, float b) { return (b > a) ? b : a; }
 __device__ __forceinline__ float il_min(float a, float b) { return (b < a) ? b : a; }
 
-__global__ __launch_bounds__(kIlBlock) void k_il_zero(int* p, long n) {
+__global__ __launch_bounds__(kIlBlock) void k_il_zero(long long* p, long n) {
   const long q = (long)blockIdx.x * kIlBlock + threadIdx.x;
   if (q < n) p[q] = 0;
 }
@@ -79,9 +79,9 @@ __global__ __launch_bounds__(kIlBlock) void k_il_mark(IlmcArgs A, int fill) {
     if (o == 0.f && u == 0.f) continue;
     const int t = t0 + di;
     if (!fill) {
-      atomicAdd(&A.cnt[t], 1);
+      atomicAdd(&A.cnt[t], 1ull);
     } else {
-      const int pos = atomicAdd(&A.cnt[t], 1);
+      const unsigned long long pos = atomicAdd(&A.cnt[t], 1ull);
       A.list[pos] = (uint64_t)di | ((uint64_t)dj << 16) | ((uint64_t)dk << 32) | ((uint64_t)n << 44);
     }
   }
@@ -90,23 +90,23 @@ __global__ __launch_bounds__(kIlBlock) void k_il_mark(IlmcArgs A, int fill) {
 // One block: off = exclusive prefix sum of cnt, off[nplanes] = total; cnt becomes the fill
 // cursor (= off).
 __global__ __launch_bounds__(kIlScan) void k_il_scan(IlmcArgs A) {
-  __shared__ int s[kIlScan];
+  __shared__ long long s[kIlScan];
   const int t = threadIdx.x;
-  int run = 0;
+  long long run = 0;
   for (int b0 = 0; b0 < A.nplanes; b0 += kIlScan) {
     const int q = b0 + t;
-    const int v = q < A.nplanes ? A.cnt[q] : 0;
+    const long long v = q < A.nplanes ? (long long)A.cnt[q] : 0;
     s[t] = v;
     __syncthreads();
     for (int o = 1; o < kIlScan; o <<= 1) {
-      const int w = t >= o ? s[t - o] : 0;
+      const long long w = t >= o ? s[t - o] : 0;
       __syncthreads();
       s[t] += w;
       __syncthreads();
     }
     if (q < A.nplanes) {
       A.off[q] = run + s[t] - v;
-      A.cnt[q] = run + s[t] - v;
+      A.cnt[q] = (unsigned long long)(run + s[t] - v);
     }
     run += s[kIlScan - 1];
     __syncthreads();
@@ -115,6 +115,9 @@ __global__ __launch_bounds__(kIlScan) void k_il_scan(IlmcArgs A) {
 }
 
 // Sweep s of the visit of cell c (wave-wide; lane 0 holds the cell's current value).
+// Returns 1 if the sweep moved mass, plus 2 if the cell is back in bounds: every later sweep
+// then computes m_use = min(m_sweep, 0) <= 0 and moves nothing (ilmc_lam_loop.inc:47,51),
+// so the visit may stop there.
 template <int s, bool AB>
 __device__ __forceinline__ int il_sweep(const IlmcArgs& A, const IlmcTr& T, long c, int di, int dj, int dk,
                                         bool over, float& fc, float& o, float& u, float mnc, float mxc, float airc,
@@ -171,17 +174,19 @@ __device__ __forceinline__ int il_sweep(const IlmcArgs& A, const IlmcTr& T, long
     }
     bc[0] = ratio;
     bc[1] = mv;
+    bc[2] = ((over ? o : u) == 0.f) ? 1.f : 0.f;
   }
   __syncthreads();
   const float ratio = bc[0];
   const int moved = bc[1] != 0.f;
+  const int done = bc[2] != 0.f;
   if (moved) {
 #pragma unroll
     for (int r = 0; r < NQ; r++)
       if (sw[r] > 0.f) T.f[qq[r]] = over ? il_min(fq[r] + sw[r] * ratio, bq[r]) : il_max(fq[r] - sw[r] * ratio, bq[r]);
   }
   __syncthreads();  // buf and bc are reused by the next sweep
-  return moved;
+  return moved | (done << 1);
 }
 
 // The visits of hyperplane t: one wave per cell, waves stride over the bucket.
@@ -189,9 +194,9 @@ template <int S, bool AB>
 __global__ __launch_bounds__(64) void k_il_plane(IlmcArgs A, int t) {
   constexpr int W = 2 * S + 1, P = AB ? W * W : W * W * W;
   __shared__ float buf[P];
-  __shared__ float bc[2];
-  const int beg = A.off[t], end = A.off[t + 1];
-  for (int e = beg + (int)blockIdx.x; e < end; e += (int)gridDim.x) {
+  __shared__ float bc[3];
+  const long long beg = A.off[t], end = A.off[t + 1];
+  for (long long e = beg + blockIdx.x; e < end; e += gridDim.x) {
     const uint64_t code = A.list[e];
     const int di = (int)(code & 0xffff), dj = (int)((code >> 16) & 0xffff), dk = (int)((code >> 32) & 0xfff);
     const int n = (int)(code >> 44);
@@ -202,10 +207,26 @@ __global__ __launch_bounds__(64) void k_il_plane(IlmcArgs A, int t) {
     float o = il_max(0.f, fc - mxc), u = il_max(0.f, mnc - fc);  // :23-26
     if (o == 0.f && u == 0.f) continue;                          // put back in bounds by earlier visits
     const bool over = o > 0.f;
-    int moved = il_sweep<1, AB>(A, T, c, di, dj, dk, over, fc, o, u, mnc, mxc, airc, buf, bc);
-    if constexpr (S >= 2) moved += il_sweep<2, AB>(A, T, c, di, dj, dk, over, fc, o, u, mnc, mxc, airc, buf, bc);
-    if constexpr (S >= 3) moved += il_sweep<3, AB>(A, T, c, di, dj, dk, over, fc, o, u, mnc, mxc, airc, buf, bc);
-    if constexpr (S >= 4) moved += il_sweep<4, AB>(A, T, c, di, dj, dk, over, fc, o, u, mnc, mxc, airc, buf, bc);
+    int r = il_sweep<1, AB>(A, T, c, di, dj, dk, over, fc, o, u, mnc, mxc, airc, buf, bc);
+    int moved = r & 1;
+    if constexpr (S >= 2) {
+      if (!(r & 2)) {
+        r = il_sweep<2, AB>(A, T, c, di, dj, dk, over, fc, o, u, mnc, mxc, airc, buf, bc);
+        moved += r & 1;
+      }
+    }
+    if constexpr (S >= 3) {
+      if (!(r & 2)) {
+        r = il_sweep<3, AB>(A, T, c, di, dj, dk, over, fc, o, u, mnc, mxc, airc, buf, bc);
+        moved += r & 1;
+      }
+    }
+    if constexpr (S >= 4) {
+      if (!(r & 2)) {
+        r = il_sweep<4, AB>(A, T, c, di, dj, dk, over, fc, o, u, mnc, mxc, airc, buf, bc);
+        moved += r & 1;
+      }
+    }
     if (A.counts && threadIdx.x == 0 && moved) atomicAdd((unsigned long long*)&A.counts[3 * n], (unsigned long long)moved);
   }
 }
@@ -289,10 +310,10 @@ int adz_ilmc_create(const adz_ilmc_tracer* tr, int ntr, const adz_ilmc_domain* d
   a.b = d.autobar ? 0 : a.a * a.a;
   const long np = (long)(ni - 1) + (long)a.a * (nj - 1) + (long)a.b * (nkk - 1) + 1;
   const long cap = (long)ntr * ni * nj * nkk;
-  if (np > (1L << 30) || cap > (1L << 31) - 1) return fail(ADZ_EINVAL, "adz_ilmc_create: domain too large");
+  if (np > (1L << 30)) return fail(ADZ_EINVAL, "adz_ilmc_create: domain too large");
   a.nplanes = (int)np;
   const size_t tb = (sizeof(IlmcTr) * ntr + 255) / 256 * 256;
-  const size_t cb = (sizeof(int) * np + 255) / 256 * 256, ob = (sizeof(int) * (np + 1) + 255) / 256 * 256;
+  const size_t cb = (sizeof(long long) * np + 255) / 256 * 256, ob = (sizeof(long long) * (np + 1) + 255) / 256 * 256;
   const size_t lb = sizeof(uint64_t) * cap;
   void* mem = nullptr;
   if (hipMalloc(&mem, tb + cb + ob + lb) != hipSuccess) return fail(ADZ_ENOMEM, "adz_ilmc_create: hipMalloc");
@@ -302,16 +323,16 @@ int adz_ilmc_create(const adz_ilmc_tracer* tr, int ntr, const adz_ilmc_domain* d
   }
   char* p = (char*)mem;
   a.tr = (const IlmcTr*)p;
-  a.cnt = (int*)(p + tb);
-  a.off = (int*)(p + tb + cb);
+  a.cnt = (unsigned long long*)(p + tb);
+  a.off = (long long*)(p + tb + cb);
   a.list = (uint64_t*)(p + tb + cb + ob);
   adz_ilmc* r = new adz_ilmc;
   r->a = a;
   r->sweep_max = d.sweep_max; r->autobar = d.autobar != 0; r->min_max = d.min_max != 0;
-  // waves per hyperplane launch: enough for the busiest buckets when ~1/8 of the cells
-  // violate, bounded to a few per CU
-  const long want = cap / np / 8;
-  r->plane_blocks = (int)std::min(2048L, std::max(64L, want));
+  // waves per hyperplane launch: enough for the busiest buckets when ~1/4 of the cells
+  // violate, bounded to 32 per CU
+  const long want = cap / np / 4;
+  r->plane_blocks = (int)std::min(8192L, std::max(64L, want));
   r->mem = mem;
   *out = r;
   return ADZ_OK;
@@ -323,11 +344,11 @@ int adz_ilmc_lam(adz_ilmc* il, const void* air_m, long long* counts, adz_stream 
   IlmcArgs a = il->a;
   a.air = (const float*)air_m;
   a.counts = counts;
-  if (counts)  // 3 * ntr int64 = 6 * ntr int32 zeros
-    hipLaunchKernelGGL(k_il_zero, dim3((unsigned)((6L * a.ntr + kIlBlock - 1) / kIlBlock)), dim3(kIlBlock), 0, st,
-                       (int*)counts, 6L * a.ntr);
-  hipLaunchKernelGGL(k_il_zero, dim3((unsigned)((a.nplanes + kIlBlock - 1) / kIlBlock)), dim3(kIlBlock), 0, st, a.cnt,
-                     (long)a.nplanes);
+  if (counts)
+    hipLaunchKernelGGL(k_il_zero, dim3((unsigned)((3L * a.ntr + kIlBlock - 1) / kIlBlock)), dim3(kIlBlock), 0, st,
+                       counts, 3L * a.ntr);
+  hipLaunchKernelGGL(k_il_zero, dim3((unsigned)((a.nplanes + kIlBlock - 1) / kIlBlock)), dim3(kIlBlock), 0, st,
+                     (long long*)a.cnt, (long)a.nplanes);
   const dim3 gr((unsigned)(a.nj * a.nkk), (unsigned)a.ntr);
   hipLaunchKernelGGL(k_il_mark, gr, dim3(kIlBlock), 0, st, a, 0);
   hipLaunchKernelGGL(k_il_scan, dim3(1), dim3(kIlScan), 0, st, a);

@@ -13,6 +13,8 @@
 !                                    adz_tricub_hlt with F_post on a LAM with Adz_BC_LAM_flux = 1
 !   adz_BC_LAM_zlf_hlt.F90:18-319, adz_BC_LAM_zlf_0_hlt.F90:18-102 -> adz_BC_LAM_zlf_hlt(F1, F2, F_n,
 !                                    F_setup), adz_BC_LAM_zlf_0_hlt(F_nptr, F_number)
+!   ILMC_LAM.F90:18-273         -> ILMC_LAM(F_ns, F_name_S, F_i0, F_in, F_j0, F_jn), called by
+!                                    adz_post_tr(1) for the tracers whose mono code is 2
 !
 ! The module variables stand in for what the reference keeps in adz_mem / adz_options /
 ! dyn_fisl_options / cstv / tr3d (same names).  Array-valued members that the reference
@@ -25,8 +27,8 @@
 !   * With F_post present, tracers whose meta has mono == 1 are clipped to [min, max]
 !     inside the interpolation launch (the same operation adz_post_tr.F90:81-94 applies to
 !     the same box afterwards), so adz_post_tr(1) only registers the fixer's tracers.
-!   * ILMC (mono = 2) and the fixer of the BICHQV group (adz_post_tr(2)) are not part of
-!     this library: they stop with a message.  Quintic vertical interpolation (F_QV_L, FP32
+!   * The fixer of the BICHQV group (adz_post_tr(2)) is not part of this library: it stops
+!     with a message.  ILMC (mono = 2), quintic vertical interpolation (F_QV_L, FP32
 !     fields) and the ZLF boundary treatment (Adz_BC_LAM_flux = 2: adz_BC_LAM_zlf_0_hlt
 !     around the interpolation on Adz_pb, the fixer on the Adz_pb box) are.
 !   * Bermejo_Conde synchronises the device to return each tracer's mass_deficit in
@@ -85,6 +87,9 @@ module adz_gpu
    integer :: Schm_advec = 1
    real(kind=REAL64) :: Cstv_bA_m_8 = 0.6d0, Cstv_bA_8 = 0.6d0   ! GEM's default off-centring
    logical :: Adz_BC_min_max_L = .true.
+   integer :: Adz_ILMC_sweep_max = 2             ! adz_options.F90:47-53
+   logical :: Adz_ILMC_min_max_L = .true.
+   logical :: Schm_autobar_L = .false.
    real :: Adz_BC_pexp_list(9) = 1.0
    integer :: Adz_pos_dtype = ADZ_F32       ! storage of positions: ADZ_F32 (the reference's real) / ADZ_F64
    integer :: Adz_field_dtype = ADZ_F32     ! storage of fields (the reference's real)
@@ -202,10 +207,6 @@ contains
          cnt = 0
          do n = 1, F_nptr
             if (present(F_post)) then
-               if (F_post(n)%mono == 2) then
-                  write (error_unit, *) 'adz_tricub_hlt: ILMC (mono=2) is not part of this library'
-                  error stop 1
-               end if
                if ((F_post(n)%mono == 1) .neqv. (pass == 2)) cycle
             end if
             cnt = cnt + 1
@@ -319,9 +320,10 @@ contains
       end if
    end subroutine adz_BC_LAM_zlf_0_hlt
 
-   ! F_wp = 1: register the Tr_3CWP tracers that use the fixer (their clipping happened in
-   ! adz_tricub_hlt); F_wp = 0: core limits, settings from each tracer's mass code,
-   ! Bermejo_Conde, and the carried deficits back into Tr_3CWP.
+   ! F_wp = 1: ILMC on the Tr_3CWP tracers whose mono code is 2 (adz_post_tr.F90:104), then
+   ! register those that use the fixer (the clipping of mono = 1 happened in adz_tricub_hlt);
+   ! F_wp = 0: core limits, settings from each tracer's mass code, Bermejo_Conde, and the
+   ! carried deficits back into Tr_3CWP.
    subroutine adz_post_tr(F_wp)
       integer, intent(in) :: F_wp
       integer :: n, num, i0, in, j0, jn, k0
@@ -336,6 +338,8 @@ contains
          allocate (Adz_bc(size(Tr_3CWP)))
          n_bc_TOTAL = 0
          do n = 1, size(Tr_3CWP)
+            if (Tr_3CWP(n)%mono == 2) call ILMC_LAM(n, 'TR/'//trim(Tr_3CWP(n)%name)//':M', Adz_i0, Adz_in, &
+                                                    Adz_j0, Adz_jn)
             bc_L = Tr_3CWP(n)%mass == 1 .or. (Tr_3CWP(n)%mass >= 111 .and. Tr_3CWP(n)%mass <= 139)
             if (.not. bc_L) cycle
             n_bc_TOTAL = n_bc_TOTAL + 1
@@ -371,6 +375,27 @@ contains
          Tr_3CWP(Adz_bc(num)%n)%BC_mass_deficit = Adz_bc(num)%mass_deficit
       end do
    end subroutine adz_post_tr
+
+   ! ILMC_LAM.F90:18-273 on Adz_stack(F_ns)%dst with Adz_post(F_ns)%min / %max and the air
+   ! mass at time M (airm0), over F_i0:F_in x F_j0:F_jn x Adz_k0t:l_nk of a tile that touches
+   ! the four LAM edges.  F_name_S names the tracer in messages.
+   subroutine ILMC_LAM(F_ns, F_name_S, F_i0, F_in, F_j0, F_jn)
+      character(len=*), intent(in) :: F_name_S
+      integer, intent(in) :: F_ns, F_i0, F_in, F_j0, F_jn
+      type(adz_ilmc_tracer) :: tr(1)
+      type(adz_ilmc_domain) :: dom
+      type(c_ptr) :: h
+      if (.not. c_associated(airm0)) then
+         write (error_unit, *) 'ILMC_LAM: airm0 must be set before ', trim(F_name_S)
+         error stop 1
+      end if
+      tr(1) = adz_ilmc_tracer(Adz_stack(F_ns)%dst, Adz_post(F_ns)%min, Adz_post(F_ns)%max)
+      dom = adz_ilmc_domain(l_minx, l_maxx, l_miny, l_maxy, l_nk, F_i0, F_in, F_j0, F_jn, Adz_k0t, &
+                            Adz_ILMC_sweep_max, merge(1, 0, Adz_ILMC_min_max_L), merge(1, 0, Schm_autobar_L))
+      call adz_check(adz_ilmc_create(tr, 1_c_int, dom, h), 'ILMC_LAM ('//trim(F_name_S)//'): adz_ilmc_create')
+      call adz_check(adz_ilmc_lam(h, airm0, c_null_ptr, Adz_stream), 'ILMC_LAM ('//trim(F_name_S)//')')
+      call adz_check(adz_ilmc_free(h), 'ILMC_LAM: adz_ilmc_free')
+   end subroutine ILMC_LAM
 
    subroutine Bermejo_Conde(F_bc, F_ntr_bc, F_i0, F_in, F_j0, F_jn, F_k0)
       integer, intent(in) :: F_i0, F_in, F_j0, F_jn, F_k0, F_ntr_bc

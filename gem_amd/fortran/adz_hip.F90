@@ -71,6 +71,16 @@ module adz_hip
       integer(c_int) :: ext, k0
    end type adz_zlf_domain
 
+   type, bind(C) :: adz_ilmc_tracer
+      type(c_ptr) :: f = c_null_ptr, mn = c_null_ptr, mx = c_null_ptr
+   end type adz_ilmc_tracer
+
+   type, bind(C) :: adz_ilmc_domain
+      integer(c_int) :: minx, maxx, miny, maxy, nk
+      integer(c_int) :: i0, in, j0, jn, k0
+      integer(c_int) :: sweep_max, min_max, autobar
+   end type adz_ilmc_domain
+
    type, bind(C) :: adz_flux_domain
       integer(c_int) :: minx, maxx, miny, maxy, nk
       integer(c_int) :: ni, nj
@@ -319,6 +329,24 @@ module adz_hip
          type(c_ptr), intent(in) :: f1(*), f2(*)
          type(adz_zlf_domain), intent(in) :: dom
          type(c_ptr), value :: stream
+         integer(c_int) :: rc
+      end function
+      function adz_ilmc_create(tr, ntr, dom, il) bind(C, name="adz_ilmc_create") result(rc)
+         import :: c_int, c_ptr, adz_ilmc_tracer, adz_ilmc_domain
+         type(adz_ilmc_tracer), intent(in) :: tr(*)
+         integer(c_int), value :: ntr
+         type(adz_ilmc_domain), intent(in) :: dom
+         type(c_ptr), intent(out) :: il
+         integer(c_int) :: rc
+      end function
+      function adz_ilmc_lam(il, air_m, counts, stream) bind(C, name="adz_ilmc_lam") result(rc)
+         import :: c_int, c_ptr
+         type(c_ptr), value :: il, air_m, counts, stream
+         integer(c_int) :: rc
+      end function
+      function adz_ilmc_free(il) bind(C, name="adz_ilmc_free") result(rc)
+         import :: c_int, c_ptr
+         type(c_ptr), value :: il
          integer(c_int) :: rc
       end function
       function adz_flux_free(fx) bind(C, name="adz_flux_free") result(rc)

@@ -7,7 +7,8 @@ tracers, Adz_post lin/min/max, fixed tracers and the carried mass deficits (the 
 the LAM boundary flux, Adz_BC_LAM_flux = 1).  Both interpolation forms: tricubic, and
 bicubic-horizontal / quintic-vertical (F_QV_L = .true.); and the ZLF boundary treatment
 (Adz_BC_LAM_flux = 2: interpolation on Adz_pb, piloting values saved and restored, the
-fixer on the Adz_pb box)."""
+fixer on the Adz_pb box); and ILMC (tracers with mono = 2, ILMC_LAM called by
+adz_post_tr(1) before the fixer)."""
 import os
 import subprocess
 
@@ -22,7 +23,7 @@ DRIVER = os.path.join(ROOT, "gem_amd", "fortran", "adz_f_driver")
 MH = 2
 
 
-@pytest.mark.parametrize("form", ["tricubic", "quintic", "zlf"])
+@pytest.mark.parametrize("form", ["tricubic", "quintic", "zlf", "ilmc"])
 def test_fortran_step_matches_c_abi(cuda, tmp_path, form):
     if not os.path.exists(DRIVER):
         pytest.fail("gem_amd/fortran/adz_f_driver is not built")
@@ -42,7 +43,7 @@ def test_fortran_step_matches_c_abi(cuda, tmp_path, form):
     trc = [S.plume_tracer(mod, rng, "cpu").float().numpy().reshape(-1) for _ in range(4)]
     airm1, airm0, pkps, mask = S.airmass(cfg, g, mh=MH)
     mass = np.array([1, 112, 121, 0], np.int32)
-    mono = np.array([1, 0, 1, 0], np.int32)
+    mono = np.array([2, 0, 2, 0] if form == "ilmc" else [1, 0, 1, 0], np.int32)
     names = ["uu_ext", "vv_ext", "ww_ext", "ut0", "vt0", "zdt0"]
     for k in names:
         assert w[k].numel() == n_ext and w[k].dtype == torch.float32, k
@@ -65,7 +66,7 @@ def test_fortran_step_matches_c_abi(cuda, tmp_path, form):
         mass.tofile(f)
         mono.tofile(f)
     print("[gpu-case] running the Fortran driver", flush=True)
-    mode = {"tricubic": [], "quintic": ["qv"], "zlf": ["zlf"]}[form]
+    mode = {"tricubic": [], "quintic": ["qv"], "zlf": ["zlf"], "ilmc": []}[form]
     r = subprocess.run([DRIVER, str(fin), str(fout)] + mode, capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
 
@@ -103,6 +104,15 @@ def test_fortran_step_matches_c_abi(cuda, tmp_path, form):
         else:
             A.adz_tricub_hlt(grp, ctx.array(A.ADZ_A_PT), ctx.ztab(True), *box[:5], box[5], dl, ext=False,
                              post=post, math=L.ADZ_MATH_REF, src_off=off)
+    d_air0, d_mask = airm0.reshape(-1).cuda(), mask.reshape(-1).double().cuda()
+    if form == "ilmc":  # adz_post_tr(1): ILMC_LAM on Adz_i0:Adz_in x Adz_j0:Adz_jn x Adz_k0t:nk
+        for n in np.nonzero(mono == 2)[0]:
+            e = ent[n]
+            assert int(((e["dst"] < e["min"]) | (e["dst"] > e["max"])).sum()) > 0, n  # ILMC has work to do
+            il = A.IlmcLam([dict(f=ent[n]["dst"], min=ent[n]["min"], max=ent[n]["max"])], dl + (nk,), box[:5], off)
+            il(d_air0)
+            torch.cuda.synchronize()
+            il.free()
     bcn = [n for n in range(4) if mass[n] == 1 or 111 <= mass[n] <= 139]
     bcs = [dict(p=ent[n]["src"], m=ent[n]["dst"], lin=ent[n]["lin"], min=ent[n]["min"], max=ent[n]["max"],
                 mass=int(mass[n])) for n in bcn]
@@ -112,7 +122,6 @@ def test_fortran_step_matches_c_abi(cuda, tmp_path, form):
         scope = (i["i0b"], i["inb"], i["j0b"], i["jnb"], 1)
     bc = A.BermejoConde(bcs, dl + (nk,), scope, off, min_max=True)
     deficit = torch.zeros(len(bcn), dtype=torch.float64, device="cuda")
-    d_air0, d_mask = airm0.reshape(-1).cuda(), mask.reshape(-1).double().cuda()
     fx = None
     if form == "zlf":
         bc(airm1.reshape(-1).cuda(), d_air0, d_mask, deficit, pkps=pkps.reshape(-1).cuda())
