@@ -293,7 +293,9 @@ int adz_ilmc_create(const adz_ilmc_tracer* tr, int ntr, const adz_ilmc_domain* d
   if (d.sweep_max < 1 || d.sweep_max > 4)
     return fail(ADZ_EINVAL, "adz_ilmc_create: ILMC_sweep_max must be 1..4 (the sweep arrays hold 400 cells)");
   const int ni = d.in - d.i0 + 1, nj = d.jn - d.j0 + 1, nkk = d.nk - d.k0 + 1;
-  if (ni > 65535 || nj > 65535 || nkk > 4095 || ntr > 65535)
+  // cell offsets are packed in 16 + 16 + 12 bits, tracers in the remaining 20; the row
+  // kernels launch one 256-lane block per row of the domain (at most 2^32 lanes)
+  if (ni > 65535 || nj > 65535 || nkk > 4095 || ntr > 65535 || (long)nj * nkk > (1L << 24))
     return fail(ADZ_EINVAL, "adz_ilmc_create: domain or tracer count too large");
   std::vector<IlmcTr> h(ntr);
   for (int n = 0; n < ntr; n++) {

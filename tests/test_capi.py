@@ -52,3 +52,31 @@ def test_missing_library_fails_loudly(monkeypatch, tmp_path):
     monkeypatch.setattr(L, "_lib", None)
     with pytest.raises(L.AdzError):
         L.lib()
+
+
+@pytest.mark.parametrize("case", ["sweep0", "sweep5", "outside", "null", "rows"])
+def test_ilmc_create_refuses_bad_arguments(case):
+    """adz_ilmc_create validates before any device call (safe without a GPU)."""
+    if not os.path.exists(L.LIB_PATH):
+        pytest.skip("libadz_hip.so not built")
+    from gem_amd import adz as A
+
+    dom = dict(minx=-1, maxx=12, miny=-1, maxy=10, nk=8, i0=1, in_=10, j0=1, jn=8, k0=2, sweep_max=2, min_max=1,
+               autobar=0)
+    tr = (A.AdzIlmcTracer * 1)()
+    tr[0].f, tr[0].mn, tr[0].mx = 4096, 8192, 12288  # never dereferenced: validation fails first
+    if case == "sweep0":
+        dom["sweep_max"] = 0
+    elif case == "sweep5":
+        dom["sweep_max"] = 5
+    elif case == "outside":
+        dom["in_"] = 13
+    elif case == "null":
+        tr[0].mx = None
+    else:
+        dom.update(maxy=70000, jn=65535, nk=300, k0=1)
+    d = A.AdzIlmcDomain(**dom)
+    h = ctypes.c_void_p()
+    assert L.lib().adz_ilmc_create(tr, 1, ctypes.byref(d), ctypes.byref(h)) == 1
+    assert "adz_ilmc_create" in L.lib().adz_last_error().decode()
+    assert not h.value
