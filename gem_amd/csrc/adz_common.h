@@ -187,7 +187,7 @@ __device__ __forceinline__ void tricub3_ref(const TF* __restrict__ f, const doub
   // rows outer, the 12 interleaved values of a row inner: only one row's values and the
   // 12 running sums are live at a time (each element keeps the same operation order)
   double dst[12];
-#pragma unroll
+#pragma unroll 1
   for (int r = 0; r < 4; r++) {
     const TF* s = f + r * ni3;
     double za[12];
@@ -199,8 +199,9 @@ __device__ __forceinline__ void tricub3_ref(const TF* __restrict__ f, const doub
     for (int e = 0; e < 12; e++) za[e] = fma(to_f64(s[e + o2]), c[10], za[e]);
 #pragma unroll
     for (int e = 0; e < 12; e++) za[e] = fma(to_f64(s[e + o3]), c[11], za[e]);
+    const double cyr = r == 0 ? c[4] : (r == 1 ? c[5] : (r == 2 ? c[6] : c[7]));  // no dynamic index into c
 #pragma unroll
-    for (int e = 0; e < 12; e++) dst[e] = (r == 0) ? za[e] * c[4] : fma(za[e], c[4 + r], dst[e]);
+    for (int e = 0; e < 12; e++) dst[e] = (r == 0) ? za[e] * cyr : fma(za[e], cyr, dst[e]);
   }
 #pragma unroll
   for (int q = 0; q < 3; q++) {
