@@ -353,6 +353,18 @@ def main():
                   24 * (npts(boxes["u"][0]) + npts(boxes["v"][0]) + Nt))
     interp_gpts = pts_total / (t_interp * 1e-3) / 1e9
 
+    # one tracer's fixer inputs (after its group's interpolation) for the CPU baseline
+    bc_cpu_in = None
+    if not args.no_cpu and groups:
+        ids0_, plan0_, _ = groups[0]
+        plan0_.execute()
+        torch.cuda.synchronize()
+        q0 = ids0_[0]
+        bc_cpu_in = dict(p=src[q0].cpu().numpy(), m=dst[q0].cpu().numpy(), lin=post[0]["lin"].cpu().numpy(),
+                         min=post[0]["min"].cpu().numpy(), max=post[0]["max"].cpu().numpy(),
+                         air1=airm1.cpu().numpy(), air0=airm0.cpu().numpy(), mask=mask.cpu().numpy(),
+                         flux=fmass[2 * (q0 - 4):2 * (q0 - 4) + 2].cpu().numpy() if fx else None)
+
     # ------------------------------------------------------------------ accuracy vs FP64
     interp()  # dst now holds the pure FP32 interpolation (no fixer)
     torch.cuda.synchronize()
@@ -403,9 +415,15 @@ def main():
     torch.cuda.empty_cache()
 
     # ------------------------------------------------------------------ CPU baseline
-    cpu = None
+    cpu = bc_cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(np, torch, cfg, g, src, pt, tbox, mod_shape, nf, args.cpu_seconds)
+        if bc_cpu_in is not None:
+            bc_cpu = bc_cpu_baseline(np, bc_cpu_in, layout5, scope)
+            bc_cpu_in = None
+            log(f"fixer CPU baseline (the oracle's Bermejo_Conde in the reference's sequential order, 1 thread): "
+                f"{bc_cpu['cpu_ms_per_tracer_reference_order']:.0f} ms per tracer ({bc_cpu['sample']}); "
+                f"GPU {t_bc / max(ntr, 1):.2f} ms per tracer")
         if ilmc_cpu_in is not None:
             ilmc["ref_math"].update(ilmc_cpu_baseline(np, ilmc_cpu_in, ilmc_gpu_out, layout5, tbox[:5]))
             log(f"ILMC CPU baseline (the oracle's sequential loop, 1 thread, one tracer): "
@@ -441,6 +459,7 @@ def main():
                       "eager_step": round(eager_ms, 3)},
         "lam_flux_targets_interpolated": flux_active,
         "ilmc_beside_step": ilmc,
+        "fixer_cpu_baseline": bc_cpu,
         "interp_gpoints_per_s": round(interp_gpts, 3),
         "interp_alg_GBps": round(alg_interp / (t_interp * 1e-3) / 1e9, 1),
         "fp64_plain_gpoints_per_s": round(plain_gpts, 3),
@@ -633,6 +652,25 @@ def measured_traffic(config, tiles):
     if m.get("tiles") != tiles:
         return None
     return m.get("hbm_bytes_per_launch")
+
+
+def bc_cpu_baseline(np, d, layout, scope, levels=8):
+    """Bermejo_Conde of one tracer by the oracle with the reference's sequential FP64 sums,
+    one thread as in the reference (adz_tracers.F90:151-153), on a bounded sample: the top
+    `levels` levels of the scope, scaled to the whole scope."""
+    from oracle import oracle as O
+
+    i0, in_, j0, jn, k0 = scope
+    nk = layout[4]
+    ks = max(k0, nk - levels + 1)
+    tr = [dict(p=d["p"], m=d["m"].copy(), lin=d["lin"], min=d["min"], max=d["max"], mass=1)]
+    flux = None if d["flux"] is None else np.asarray(d["flux"], np.float64)
+    t0 = time.time()
+    O.bermejo_conde(tr, layout, (i0, in_, j0, jn, ks), d["air1"], d["air0"], d["mask"], np.zeros(1),
+                    min_max=True, sum_mode=0, flux=flux)
+    ms = (time.time() - t0) * 1e3 * (nk - k0 + 1) / (nk - ks + 1)
+    return {"cpu_ms_per_tracer_reference_order": round(ms, 1), "cpu_kind": "port", "cpu_cores": 1,
+            "sample": f"levels {ks}..{nk} of {k0}..{nk}, scaled"}
 
 
 def ilmc_cpu_baseline(np, inp, gpu_out, layout, box):
