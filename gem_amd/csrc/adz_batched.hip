@@ -2,8 +2,8 @@
 // src/gemdyn/src/base/adz_interp_hlt.F90:28-187) and its batched, persistent form over
 // several subdomains.
 //
-// Work is cut into tiles of 256 consecutive target points (i fastest) times a group of
-// `fields_per_pass` fields.  One block of 256 lanes runs one tile: each lane computes the
+// Work is cut into tiles of 64 x 4 target points (i fastest: four rows of 64) times a group
+// of `fields_per_pass` fields.  One block of 256 lanes runs one tile: each lane computes the
 // index/weights of its target once and interpolates the group's fields, writing straight
 // into the destination arrays (the reference interpolates into a work buffer and scatters
 // it, :165-180).  A persistent grid walks the tile list; with `xcd_chunks` the tile list is
@@ -20,7 +20,7 @@
 namespace adz {
 
 constexpr int kTI = 64;       // targets per tile along i (one wavefront)
-constexpr int kTJ = 4;        // rows of targets per tile (four wavefronts)
+constexpr int kTJ = 4;        // rows of targets per tile (one wavefront each)
 constexpr int kTile = kTI * kTJ;
 constexpr int kPitch = 128;       // LDS row pitch (elements): a box row spans at most 128 columns
 constexpr int kMaxRows = 39;      // box rows (y x planes) per LDS buffer; two buffers per block
@@ -168,17 +168,22 @@ __device__ __forceinline__ void store_nan(const Fields& F, int f, long o) {
   if (POST & ADZ_POST_STORE) { ((typename GPtr<TF>::type)F.l(f))[o] = nan; ((typename GPtr<TF>::type)F.n(f))[o] = nan; ((typename GPtr<TF>::type)F.x(f))[o] = nan; }
 }
 
-// block-wide minimum of 6 values over 256 lanes (4 waves of 64); `red` = 24 ints of LDS
+// block-wide minimum of 6 values over the tile's kTJ waves of 64; `red` = 6 * kTJ ints of LDS
 __device__ __forceinline__ void block_min6(int* v, int* red) {
 #pragma unroll
   for (int q = 0; q < 6; q++)
     for (int s = 32; s > 0; s >>= 1) v[q] = min(v[q], __shfl_xor(v[q], s, 64));
   if ((threadIdx.x & 63) == 0)
 #pragma unroll
-    for (int q = 0; q < 6; q++) red[q * 4 + (threadIdx.x >> 6)] = v[q];
+    for (int q = 0; q < 6; q++) red[q * kTJ + (threadIdx.x >> 6)] = v[q];
   __syncthreads();
 #pragma unroll
-  for (int q = 0; q < 6; q++) v[q] = min(min(red[q * 4], red[q * 4 + 1]), min(red[q * 4 + 2], red[q * 4 + 3]));
+  for (int q = 0; q < 6; q++) {
+    int m = red[q * kTJ];
+#pragma unroll
+    for (int w = 1; w < kTJ; w++) m = min(m, red[q * kTJ + w]);
+    v[q] = m;
+  }
   __syncthreads();
 }
 
@@ -371,11 +376,11 @@ __device__ __forceinline__ void run_tile(const DevItem& it, const Fields& F, lon
 // LDS slots with 16-byte buffer-to-LDS DMA, several fields ahead.  It issues no stores,
 // so its vector-memory counter counts its own DMA in order, and s_waitcnt vmcnt(k*R)
 // tells it when an older field has landed; it then publishes that field in an LDS flag.
-// Waves 1..4 (one target row each) wait for the flag, read their stencils from LDS,
+// Waves 1..kTJ (one target row each) wait for the flag, read their stencils from LDS,
 // evaluate, store, and return the slot with an LDS atomic.  Every wait is a bounded spin
 // (it gives up after about 2^22 sleeps instead of hanging), and one block barrier per
 // tile separates tiles.
-constexpr int kPCWaves = 5;
+constexpr int kPCWaves = kTJ + 1;
 constexpr int kPCThreads = kPCWaves * 64;
 constexpr int kRing = 12288;   // floats of slot ring per block (48 KB)
 constexpr int kMaxSlots = 8;
@@ -497,7 +502,7 @@ __device__ __forceinline__ void pc_tile(const DevItem& it, const ItemFields& F, 
     const float rcpr = 1.0f / (float)b.cpr, rnyb = 1.0f / (float)nyb;
     for (int q = 0; q < nf; q++) {
       const int sl = q % b.ns;
-      if (q >= b.ns) spin_ge(&freed[sl], 4 * (q / b.ns), abort_flag);
+      if (q >= b.ns) spin_ge(&freed[sl], kTJ * (q / b.ns), abort_flag);
       const float* base = (const float*)F.sc(f0 + q);
       const int sh = (int)((((unsigned long long)base >> 2) + (unsigned)(b.x0 - 1)) & 3);
       const __amdgpu_buffer_rsrc_t rs = field_rsrc(base + minoff, span);
@@ -623,7 +628,7 @@ __global__ __launch_bounds__(kTile) void k_plan(PlanArgs A) {
   if constexpr (STAGE == 1) {
     __shared__ TF boxA[kMaxRows * kPitch];
     __shared__ TF boxB[kMaxRows * kPitch];
-    __shared__ int red[24];
+    __shared__ int red[6 * kTJ];
     __shared__ int rowoff[kMaxRows];
     plan_loop<TF, TP, MATH, POST, 1>(A, boxA, boxB, red, rowoff);
   } else {
@@ -907,7 +912,7 @@ int adz_plan_autotune(const adz_item* items, int nitems, int fdtype, int math, i
   cur.fields_per_pass = std::min(cur.fields_per_pass, maxf);
   float cur_ms = 0.f;
   int rc = time_one(cur, &cur_ms);
-  const int stages[] = {1, 2, 3}, fpps[] = {2, 4, 8, 16, 32, 64}, bpcs[] = {2, 4, 8}, flags[] = {0, 1},
+  const int stages[] = {1, 2, 3}, fpps[] = {2, 4, 8, 16, 32, 64}, bpcs[] = {2, 4, 8, 16}, flags[] = {0, 1},
             orders[] = {0, 1, 2, 4, 8};
   // coordinates: 0 staging, 1 tile order, 2 fields per pass, 3 blocks per CU, 4 loop order,
   // 5 XCD chunking; the tile order is searched again after fields per pass (they interact
@@ -916,7 +921,7 @@ int adz_plan_autotune(const adz_item* items, int nitems, int fdtype, int math, i
   for (int ci = 0; ci < 7 && rc == ADZ_OK; ci++) {
     const int coord = coords[ci];
     const int* vals = coord == 0 ? stages : coord == 1 ? orders : coord == 2 ? fpps : coord == 3 ? bpcs : flags;
-    int nv = coord == 0 ? (fdtype == ADZ_F32 ? 3 : 2) : coord == 1 ? 5 : coord == 2 ? 6 : coord == 3 ? 3 : 2;
+    int nv = coord == 0 ? (fdtype == ADZ_F32 ? 3 : 2) : coord == 1 ? 5 : coord == 2 ? 6 : coord == 3 ? 4 : 2;
     for (int v = 0; v < nv && rc == ADZ_OK; v++) {
       adz_tune t = cur;
       int* slot = coord == 0 ? &t.stage : coord == 1 ? &t.level_inner : coord == 2 ? &t.fields_per_pass
