@@ -403,8 +403,9 @@ int adz_ilmc_create(const adz_ilmc_tracer* tr, int ntr, const adz_ilmc_domain* d
  * (airm0).  The result equals the reference's sequential (k, j, i) visit bit for bit.
  * counts (DEVICE int64 [3*ntr] or NULL) receives per tracer: sweeps that moved mass
  * (reset(:,5)), min resets and max resets (reset(:,3:4)).  Tracer arrays must not
- * overlap.  No host synchronisation: capturable (one launch per hyperplane of the
- * domain, about ni + (2S+1) nj + (2S+1)^2 nk launches). */
+ * overlap, and calls on one plan must not overlap (they share its visiting lists).  No host
+ * synchronisation: capturable (one launch per hyperplane of the domain, about
+ * ni + (2S+1) nj + (2S+1)^2 nk launches). */
 int adz_ilmc_lam(adz_ilmc* il, const void* air_m, long long* counts, adz_stream stream);
 int adz_ilmc_free(adz_ilmc* il);
 
