@@ -50,6 +50,9 @@ def parse():
     p.add_argument("--no-ilmc", action="store_true", help="skip the ILMC (mono = 2) timing beside the step")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--cpu-seconds", type=float, default=12.0)
+    p.add_argument("--profile-step", action="store_true",
+                   help="run one step (each of its kernels once), then a 1 GiB copy for calibrating PMC byte "
+                        "counters, and exit (for rocprofv3 --pmc)")
     p.add_argument("--profile-kernel", action="store_true",
                    help="only build the workload and run the dominant interpolation kernel 5 times (rocprofv3)")
     p.add_argument("--traj-storage", choices=("f32", "f64"), default="f32",
@@ -220,6 +223,19 @@ def main():
         b.record()
         torch.cuda.synchronize()
         return a.elapsed_time(b) / reps
+
+    if args.profile_step:
+        step()  # warm (first-launch effects out of the counted step)
+        torch.cuda.synchronize()
+        step()
+        torch.cuda.synchronize()
+        x = torch.empty(2 ** 28, dtype=torch.float32, device=dev)
+        y = torch.empty_like(x)
+        y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"profile_step": cfg.name, "tiles": {"uvtq": tuneA, "tracer_groups": tuneB},
+                          "copy_bytes": 2 * 4 * 2 ** 28}), flush=True)
+        return
 
     if args.profile_kernel:
         ids, plan, bc = groups[0]
