@@ -103,18 +103,24 @@ __device__ __forceinline__ T bc_weight_abs(T m, T lin, double pkps, const BcTr& 
   return (T)v;
 }
 
-// Each block owns whole rows of one tracer (a fixed partition), lanes stride along i.
+// Each block owns whole rows (a fixed partition) for kBcWaves tracers, one wave per tracer,
+// lanes striding along i.  The waves of a block walk the same rows together, so the air
+// masses, mask and pkps they share are fetched once for the block and served to the other
+// waves from the CU's caches.
+constexpr int kBcWaves = kBcBlock / 64;
 template <class T, int PASS>
 __global__ __launch_bounds__(kBcBlock) void k_bc_pass(BcArgs A) {
-  // 1-D grid of ntr*nblk blocks (a multiple of 8): blocks are dispatched round-robin over
-  // the 8 XCDs, so XCD x takes the x-th eighth of the (row block, tracer) list, tracer
-  // fastest: the blocks of every tracer over the same rows share one XCD's L2 for the air
-  // mass, mask and pkps
-  const long nlog = (long)A.ntr * A.nblk, per = nlog / 8;
+  // 1-D grid of quads*nblk blocks (a multiple of 8): blocks are dispatched round-robin over
+  // the 8 XCDs, so XCD x takes the x-th eighth of the (row block, tracer group) list,
+  // tracer group fastest: the blocks over the same rows share one XCD's L2
+  const long quads = (A.ntr + kBcWaves - 1) / kBcWaves;
+  const long nlog = quads * A.nblk, per = nlog / 8;
   const long lg = (long)(blockIdx.x % 8) * per + blockIdx.x / 8;
   if (lg >= nlog) return;
-  const int n = (int)(lg % A.ntr);
-  const int by = (int)(lg / A.ntr);
+  const int lane = threadIdx.x & 63;
+  const int n = (int)(lg % quads) * kBcWaves + (int)(threadIdx.x / 64);
+  const int by = (int)(lg / quads);
+  if (n >= A.ntr) return;  // a wave without a tracer (the last group): nothing to add
   const BcTr t = A.tr[n];
   const double* sc = A.sc + n * kSc;
   const bool cyc = (PASS >= P_CORR) && sc[SC_CYCLE] != 0.0;
@@ -126,7 +132,7 @@ __global__ __launch_bounds__(kBcBlock) void k_bc_pass(BcArgs A) {
   const ADZ_GLOBAL T* air0 = (const ADZ_GLOBAL T*)A.airm0;
   const ADZ_GLOBAL double* maskg = (const ADZ_GLOBAL double*)A.mask;
   ADZ_GLOBAL T* mg = (ADZ_GLOBAL T*)t.m;
-  // Each lane gathers the inputs of kU elements (kU * 256 apart along the row) before it
+  // Each lane gathers the inputs of kU elements (kU * 64 apart along the row) before it
   // accumulates them in order, so a wave keeps kU loads per array in flight.
   constexpr int kU = ADZ_BC_UNROLL;
   const ADZ_GLOBAL T* lin = (const ADZ_GLOBAL T*)t.lin;
@@ -138,12 +144,12 @@ __global__ __launch_bounds__(kBcBlock) void k_bc_pass(BcArgs A) {
     const int j = A.j0 + r % A.ny, k = A.k0 + r / A.ny;
     const long q2r = (long)(A.i0 - 1) + (long)A.nxa * (j - 1);
     const long qr = q2r + (long)A.nxa * A.nya * (k - 1);
-    for (int ib = 0; ib < A.nx; ib += kU * kBcBlock) {
+    for (int ib = 0; ib < A.nx; ib += kU * 64) {
       T mv[kU], av[kU], bv[kU], lv[kU], nv[kU], xv[kU];
       double mk[kU], pkv[kU];
 #pragma unroll
       for (int u = 0; u < kU; u++) {
-        const int ii = ib + u * kBcBlock + (int)threadIdx.x;
+        const int ii = ib + u * 64 + lane;
         const bool in = ii < A.nx;
         const long q = qr + (in ? ii : 0), q2 = q2r + (in ? ii : 0);
         mk[u] = in ? maskg[q2] : 0.0;
@@ -163,7 +169,7 @@ __global__ __launch_bounds__(kBcBlock) void k_bc_pass(BcArgs A) {
       }
 #pragma unroll
       for (int u = 0; u < kU; u++) {
-        const int ii = ib + u * kBcBlock + (int)threadIdx.x;
+        const int ii = ib + u * 64 + lane;
         if (ii >= A.nx) break;
         ADZ_GLOBAL T* mp = mg + qr + ii;
         T m = mv[u];
@@ -199,29 +205,19 @@ __global__ __launch_bounds__(kBcBlock) void k_bc_pass(BcArgs A) {
       }
     }
   }
-  // fixed-order block tree in LDS (P_PM carries four sums, the other passes one)
+  // fixed-order wave tree (P_PM carries four sums, the other passes one)
   constexpr int NS = PASS == P_PM ? 4 : 1;
-  __shared__ double sh[2 * NS][kBcBlock];
-  const DD acc[4] = {s0, s1, s2, s3};
+  DD acc[4] = {s0, s1, s2, s3};
 #pragma unroll
-  for (int q = 0; q < NS; q++) { sh[2 * q][threadIdx.x] = acc[q].hi; sh[2 * q + 1][threadIdx.x] = acc[q].lo; }
-  __syncthreads();
-  for (int w = kBcBlock / 2; w > 0; w >>= 1) {
-    if (threadIdx.x < w) {
-#pragma unroll
-      for (int q = 0; q < NS; q++) {
-        DD a{sh[2 * q][threadIdx.x], sh[2 * q + 1][threadIdx.x]};
-        DD b{sh[2 * q][threadIdx.x + w], sh[2 * q + 1][threadIdx.x + w]};
-        DD c = dd_merge(a, b);
-        sh[2 * q][threadIdx.x] = c.hi; sh[2 * q + 1][threadIdx.x] = c.lo;
-      }
+  for (int q = 0; q < NS; q++)
+    for (int off = 32; off > 0; off >>= 1) {
+      const DD o{__shfl_down(acc[q].hi, off, 64), __shfl_down(acc[q].lo, off, 64)};
+      acc[q] = dd_merge(acc[q], o);
     }
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) {
+  if (lane == 0) {
     double* o = A.part + ((long)n * A.nblk + by) * 8;
 #pragma unroll
-    for (int q = 0; q < 2 * NS; q++) o[q] = sh[q][0];
+    for (int q = 0; q < NS; q++) { o[2 * q] = acc[q].hi; o[2 * q + 1] = acc[q].lo; }
   }
 }
 
@@ -365,7 +361,7 @@ int bc_apply(adz_bc* bc, const void* airm1, const void* airm0, const void* pkps,
   a.flux = flux;
   if (bc->has_w3 && !pkps) return fail(ADZ_EINVAL, "adz_bermejo_conde: weight 3 needs pkps");
   hipStream_t st = (hipStream_t)stream;
-  dim3 g((unsigned)((long)a.ntr * a.nblk)), b(kBcBlock);
+  dim3 g((unsigned)((long)((a.ntr + kBcWaves - 1) / kBcWaves) * a.nblk)), b(kBcBlock);
   dim3 gf((unsigned)a.ntr), bf(kFinBlock);
 #define ADZ_BC_PASSES(T)                                                       \
   hipLaunchKernelGGL((k_bc_pass<T, P_PM>), g, b, 0, st, a);                   \
