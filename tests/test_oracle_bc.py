@@ -9,7 +9,11 @@ pinned by properties a wrong term or sign would break:
     opposite sign of the deficit;
   * with the reset, the proportional fix restores the target mass;
   * the carried deficit identity (new epsilon = final mass - target);
-  * a tracer with m == lin everywhere is skipped and keeps its deficit.
+  * a tracer with m == lin everywhere is skipped and keeps its deficit;
+  * the shape of the correction: without the reset it is lambda * w, so corrections of
+    two cells stand in the ratio of |m - lin|^p * factor (weights 1, 2, 3; p = 1, 2, 3),
+    which a wrong exponent, a dropped max(0, m) or pr_k/pr_s, or a reversed sign rule
+    breaks (each planted against the same check).
 """
 import math
 
@@ -112,3 +116,81 @@ def test_sequential_and_exact_sums_agree_closely():
     s0 = O.mass(x, a0, mask, layout, scope, sum_mode=0)
     s1 = O.mass(x, a0, mask, layout, scope, sum_mode=1)
     assert abs(s0 - s1) <= 1e-14 * abs(s1)
+
+
+# ---------------------------------------------------------------- weight shape
+def _expected_weights(m, lin, D, p, weight, pkps):
+    """|m - lin|^p sign(D) times the weight kind's factor where sign(m - lin) = sign(D),
+    zero elsewhere (Bermejo_Conde.F90:112-172), from the formula's meaning."""
+    h = m.astype(np.float64) - lin.astype(np.float64)
+    same = np.sign(h) == np.sign(D)
+    w = np.where(same, np.abs(h) ** p * np.sign(D), 0.0)
+    if weight == 2:
+        w = w * np.maximum(0.0, m)
+    if weight == 3:
+        w = w * pkps
+    return w
+
+
+def _shape_matches(corr, w, scale):
+    """The correction is lambda * w for one lambda (fitted on the largest weight): zero
+    where w is zero, and within a few ulp of the stored value (`scale`, the larger of m
+    before and after) of lambda * w elsewhere -- corr = m - (m - lambda w) cancels, so
+    small corrections carry the rounding of m itself."""
+    nz = w != 0.0
+    if np.any(corr[~nz] != 0.0) or nz.sum() < 10:
+        return False
+    q = np.argmax(np.abs(w))
+    lam = corr.flat[q] / w.flat[q]
+    err = np.abs(corr - lam * w)
+    return bool(np.all(err <= 4 * np.finfo(np.float64).eps * scale + 1e-12 * np.abs(lam * w)))
+
+
+@pytest.mark.parametrize("weight", [1, 2, 3])
+@pytest.mark.parametrize("pexp_n", [1, 2, 3])
+@pytest.mark.parametrize("deficit_sign", [1, -1])
+def test_correction_shape_is_the_weight(weight, pexp_n, deficit_sign):
+    """Without the min/max reset the fixer moves m by lambda * w: the ratio of the
+    corrections of two cells is the ratio of their weights |m - lin|^p * factor
+    (factor 1, max(0, m) or pr_k/pr_s).  Checked in FP64 storage, where the stored
+    weight and correction are not rounded to FP32.  A wrong exponent, a missing
+    max(0, m) or a missing pr_k/pr_s fails the check (each planted below)."""
+    rng = np.random.default_rng(100 + 10 * weight + pexp_n)
+    layout, scope, a1, a0, mask, trs = _case(rng, np.float64, ntr=1)
+    t = trs[0]
+    # both signs of m - lin, and some negative m (weight 2 keeps those untouched)
+    t["m"] = t["lin"] * rng.uniform(0.9, 1.1, t["lin"].shape)
+    t["m"][rng.random(t["m"].shape) < 0.1] *= -1.0
+    # a carried deficit that fixes the sign of D = M_m - (M_p - eps)
+    mp = _mass(t["p"], a1, mask, layout, scope)
+    mm = _mass(t["m"], a0, mask, layout, scope)
+    eps = (mp - mm) + deficit_sign * 0.05 * abs(mm)
+    t["mass"] = 100 + 10 * weight + pexp_n
+    pkps = rng.uniform(0.1, 1.0, a0.shape)
+    before = t["m"].copy()
+    de, diag = O.bermejo_conde(trs, layout, scope, a1, a0, mask, np.array([eps]), pkps=pkps, min_max=False,
+                               pexp_list=(1.0, 2.0, 3.0) + (1.0,) * 6)
+    D = diag[0, 1] - diag[0, 0]
+    assert np.sign(D) == deficit_sign and diag[0, 4] == 0.0
+    i0, in_, j0, jn, k0 = scope
+    minx, maxx, miny, maxy, nk = layout
+    sl = (slice(k0 - 1, nk), slice(j0 - miny, jn - miny + 1), slice(i0 - minx, in_ - minx + 1))
+    corr = (before - t["m"])[sl]
+    scale = np.maximum(np.abs(before), np.abs(t["m"]))[sl]
+    w = _expected_weights(before, t["lin"], D, float(pexp_n), weight, pkps)[sl]
+    assert _shape_matches(corr, w, scale)
+    # lambda = D / sum(w * air0 * mask) (Bermejo_Conde.F90:200)
+    q = np.argmax(np.abs(w))
+    lam = corr.flat[q] / w.flat[q]
+    W = math.fsum((w * a0[sl]).ravel())
+    assert abs(lam - D / W) <= 1e-9 * abs(D / W)
+    # planted mistakes fail the same check
+    assert not _shape_matches(corr, _expected_weights(before, t["lin"], D, float(pexp_n) + 1.0, weight, pkps)[sl], scale)
+    if weight == 2 and deficit_sign < 0:  # cells with m < 0 < lin are selected only when D < 0
+        planted = _expected_weights(before, t["lin"], D, float(pexp_n), 1, pkps) * before
+        assert not _shape_matches(corr, planted[sl], scale)   # max(0, m) dropped
+    if weight == 3:
+        assert not _shape_matches(corr, _expected_weights(before, t["lin"], D, float(pexp_n), 1, pkps)[sl], scale)
+    if weight == 1:
+        flipped = _expected_weights(before, t["lin"], -D, float(pexp_n), 1, pkps)
+        assert not _shape_matches(corr, flipped[sl], scale)   # sign rule reversed
