@@ -154,6 +154,8 @@ L.extra_sigs({
     "adz_plan_create": ("i", ["vp", "i", "i", "i", "i", "vp", "vp"]),
     "adz_plan_execute": ("i", ["vp", "vp"]),
     "adz_plan_tune": ("i", ["vp", "vp"]),
+    "adz_plan_status": ("i", ["vp", "vp"]),
+    "adz_plan_set_spin_limit": ("i", ["vp", "i"]),
     "adz_plan_free": ("i", ["vp"]),
     "adz_plan_autotune": ("i", ["vp", "i", "i", "i", "i", "i", "vp", "vp", "vp"]),
 })
@@ -285,6 +287,15 @@ class Plan:
         t = AdzTune()
         L.check(L.lib().adz_plan_tune(self.h, C.byref(t)), "adz_plan_tune")
         return {k: getattr(t, k) for k, _ in AdzTune._fields_}
+
+    def status(self):
+        """Staging waits that gave up since the last call (adz_plan_status; synchronises)."""
+        n = C.c_int()
+        L.check(L.lib().adz_plan_status(self.h, C.byref(n)), "adz_plan_status")
+        return n.value
+
+    def set_spin_limit(self, polls):
+        L.check(L.lib().adz_plan_set_spin_limit(self.h, int(polls)), "adz_plan_set_spin_limit")
 
     def free(self):
         if getattr(self, "h", None) is not None and self.h.value:
@@ -532,3 +543,41 @@ def adz_bc_lam_zlf(setup, f1, f2, layout, tile, pil, ext, k0, off):
     a2 = (C.c_void_p * nf)(*[_ptr_at(t, off) for t in f2]) if f2 is not None else None
     dom = AdzZlfDomain(*layout, *tile, *pil, ext, k0)
     L.check(L.lib().adz_bc_lam_zlf(setup, a1, a2, nf, C.byref(dom), _DT[f1[0].dtype], _stream()), "adz_bc_lam_zlf")
+
+
+# ---------------------------------------------------------------------- air mass (set_post_tr_hlt)
+L.extra_sigs({
+    "adz_set_post_tr": ("i", ["vp", "vp", "d", "vp", "vp", "vp", "i", "vp"]),
+})
+
+GRAV = 9.80616  # tdpack grav_8
+
+
+class AdzAirmassFields(C.Structure):
+    _fields_ = [(n, C.c_int) for n in ("minx", "maxx", "miny", "maxy")] + \
+               [(n, C.c_void_p) for n in ("pm_p", "pm_m", "hu_p", "hu_m", "hyd_p", "hyd_m")] + \
+               [("nhyd", C.c_int)] + [(n, C.c_void_p) for n in ("pt_m", "p0_m", "area")] + \
+               [("dry_mixing_ratio", C.c_int), ("wload", C.c_int)]
+
+
+def adz_set_post_tr(ctx, layout, pm_p, pm_m, pt_m, p0_m, area, airm1, airm0, pkps=None, hu_p=None, hu_m=None,
+                    hyd_p=(), hyd_m=(), dry=False, wload=False, grav=GRAV):
+    """set_post_tr_hlt (set_post_tr_hlt.F90:44-70): airm1 / airm0 by get_air_mass_hlt at
+    times P and M and pkps = pr_t/p0 at M, on device tensors of the model layout
+    (minx:maxx, miny:maxy, .) flattened i fastest (pm_* with nk+1 levels, p0_m and the FP64
+    area 2-D).  hyd_p / hyd_m: the water-loading tracers in tracer order."""
+    minx, maxx, miny, maxy = layout[:4]
+    off = (1 - minx) + (maxx - minx + 1) * (1 - miny)
+    dt = airm1.dtype
+    _check_types(dt, pm_p=pm_p, pm_m=pm_m, pt_m=pt_m, p0_m=p0_m, airm0=airm0, pkps=pkps, hu_p=hu_p, hu_m=hu_m)
+    _check_types(torch.float64, area=area)
+    for h in list(hyd_p) + list(hyd_m):
+        _check_types(dt, hyd=h)
+    hp = (C.c_void_p * max(1, len(hyd_p)))(*[_ptr_at(h, off) for h in hyd_p])
+    hm = (C.c_void_p * max(1, len(hyd_m)))(*[_ptr_at(h, off) for h in hyd_m])
+    f = AdzAirmassFields(minx, maxx, miny, maxy, _ptr_at(pm_p, off), _ptr_at(pm_m, off), _ptr_at(hu_p, off),
+                         _ptr_at(hu_m, off), C.cast(hp, C.c_void_p), C.cast(hm, C.c_void_p), len(hyd_p),
+                         _ptr_at(pt_m, off), _ptr_at(p0_m, off), _ptr_at(area, off), 1 if dry else 0,
+                         1 if wload else 0)
+    L.check(L.lib().adz_set_post_tr(ctx.h, C.byref(f), float(grav), _ptr_at(airm1, off), _ptr_at(airm0, off),
+                                    _ptr_at(pkps, off), _DT[dt], _stream()), "adz_set_post_tr")

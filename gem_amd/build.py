@@ -10,7 +10,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libadz_hip.so")
-SOURCES = ["adz_interp.hip", "adz_traject.hip", "adz_batched.hip", "adz_bc.hip", "adz_flux.hip", "adz_qv.hip", "adz_ilmc.hip"]
+SOURCES = ["adz_interp.hip", "adz_traject.hip", "adz_batched.hip", "adz_bc.hip", "adz_flux.hip", "adz_qv.hip", "adz_ilmc.hip", "adz_airmass.hip"]
 
 # -ffp-contract=off: device FMAs are written explicitly, so operation order is fixed.
 # -packed-fp32-ops disabled: no v_pk_*_f32 instructions are generated.

@@ -53,6 +53,8 @@ struct PlanArgs {
   int xcd_chunks;
   int stage;
   int level_inner;
+  int spin_limit;   // staging mode 3: polls of one bounded wait before it gives up
+  int* status;      // device word: number of staging waits that gave up (the plan's)
 };
 
 struct InlineFields {  // kernel-argument table (adz_tricub_hlt)
@@ -378,8 +380,11 @@ __device__ __forceinline__ void run_tile(const DevItem& it, const Fields& F, lon
 // tells it when an older field has landed; it then publishes that field in an LDS flag.
 // Waves 1..kTJ (one target row each) wait for the flag, read their stencils from LDS,
 // evaluate, store, and return the slot with an LDS atomic.  Every wait is a bounded spin
-// (it gives up after about 2^22 sleeps instead of hanging), and one block barrier per
-// tile separates tiles.
+// (PlanArgs::spin_limit polls, 2^20 by default) instead of an unbounded one.  A wait that
+// gives up raises the block's abort flag and counts itself in the plan's status word; the
+// producer then stops filling slots, and every field whose slot was not published is
+// evaluated from global memory by per-lane loads, so the result stays exact (only slower).
+// One block barrier per tile separates tiles; an aborted block stages no later tile.
 constexpr int kPCWaves = kTJ + 1;
 constexpr int kPCThreads = kPCWaves * 64;
 constexpr int kRing = 12288;   // floats of slot ring per block (48 KB)
@@ -402,19 +407,23 @@ __device__ __forceinline__ void wait_vmcnt(int n) {
   }
 }
 
-// Bounded spin on an LDS word until *p >= v.  A wait that runs out (about 2^20 sleeps)
-// raises the block's abort flag, and every later wait of the block returns at once, so a
-// broken hand-off costs one timeout per block instead of one per wait.
+// Bounded spin on an LDS word until *p >= v or the block's abort flag is up.  A wait that
+// runs out (`limit` polls) raises the abort flag and adds one to the plan's status word, and
+// every later wait of the block returns at once, so a broken hand-off costs one timeout per
+// block instead of one per wait.  Returns true when *p >= v.
 typedef volatile __attribute__((address_space(3))) int lds_flag;  // LDS words read with ds_read
 typedef __attribute__((address_space(3))) float lds_float;
 
-__device__ __forceinline__ void spin_ge(lds_flag* p, int v, lds_flag* abort_flag) {
-  if (*abort_flag) return;
-  for (int it = 0; it < (1 << 20); it++) {
-    if (*p >= v) return;
+__device__ __forceinline__ bool spin_ge(lds_flag* p, int v, lds_flag* abort_flag, int limit, int* status) {
+  for (int it = 0;; it++) {
+    if (*p >= v) return true;
+    if (*abort_flag) return false;
+    if (it >= limit) break;
     __builtin_amdgcn_s_sleep(1);
   }
   *abort_flag = 1;
+  if ((threadIdx.x & 63) == 0) __hip_atomic_fetch_add(status, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return false;
 }
 
 // Publish a flag word.  Written in assembly: the compiler would otherwise wait for every
@@ -433,7 +442,7 @@ struct PCBox {  // the tile's staged box and ring geometry (uniform)
 template <int MATH, int POST, class TP>
 __device__ __forceinline__ void pc_tile(const DevItem& it, const ItemFields& F, long ptile, int f0, int f1,
                                         lds_float* ring, lds_flag* ready, lds_flag* freed, int* red,
-                                        lds_flag* abort_flag, int level_inner) {
+                                        lds_flag* abort_flag, int level_inner, int spin_limit, int* status) {
   const int wave = threadIdx.x / 64, lane = threadIdx.x & 63;
   const bool consumer = wave > 0;
   int ib, jb, kk;
@@ -489,7 +498,8 @@ __device__ __forceinline__ void pc_tile(const DevItem& it, const ItemFields& F, 
   b.slot = b.r * 256;                     // floats per slot (each instruction fills 1 KB)
   b.ns = e[0] == big ? 0 : kRing / b.slot;
   if (b.ns > kMaxSlots) b.ns = kMaxSlots;
-  const bool staged = e[0] != big && b.ns >= 2 && span && (gni & 3) == 0 && (gnij & 3) == 0 && b.r <= 21;
+  const bool staged = e[0] != big && b.ns >= 2 && span && (gni & 3) == 0 && (gnij & 3) == 0 && b.r <= 21 &&
+                      !*abort_flag;
   if (!staged) {  // per-lane loads for this tile; the producer has nothing to do
     if (consumer) per_lane<float, MATH, POST>(it, F, f0, f1, x, ok, valid, cw, o);
     __syncthreads();
@@ -500,9 +510,14 @@ __device__ __forceinline__ void pc_tile(const DevItem& it, const ItemFields& F, 
   if (!consumer) {
     // ------------------------------------------------------------------ producer
     const float rcpr = 1.0f / (float)b.cpr, rnyb = 1.0f / (float)nyb;
+    bool stopped = false;
     for (int q = 0; q < nf; q++) {
       const int sl = q % b.ns;
-      if (q >= b.ns) spin_ge(&freed[sl], kTJ * (q / b.ns), abort_flag);
+      // a slot is refilled only once its four consumers have returned it
+      if (q >= b.ns && !spin_ge(&freed[sl], kTJ * (q / b.ns), abort_flag, spin_limit, status)) {
+        stopped = true;
+        break;
+      }
       const float* base = (const float*)F.sc(f0 + q);
       const int sh = (int)((((unsigned long long)base >> 2) + (unsigned)(b.x0 - 1)) & 3);
       const __amdgpu_buffer_rsrc_t rs = field_rsrc(base + minoff, span);
@@ -526,7 +541,7 @@ __device__ __forceinline__ void pc_tile(const DevItem& it, const ItemFields& F, 
       }
     }
     wait_vmcnt(0);
-    if (lane == 0)
+    if (lane == 0 && !stopped)
       for (int q = max(0, nf - depth + 1); q < nf; q++) lds_publish(&ready[q % b.ns], q + 1);
   } else {
     // ------------------------------------------------------------------ consumers
@@ -541,7 +556,13 @@ __device__ __forceinline__ void pc_tile(const DevItem& it, const ItemFields& F, 
     }
     for (int q = 0; q < nf; q++) {
       const int sl = q % b.ns;
-      spin_ge(&ready[sl], q + 1, abort_flag);
+      // the slot holds field q once ready[sl] >= q + 1: the producer writes a slot only
+      // after its consumers returned it, so it cannot hold a later field yet
+      const bool landed = spin_ge(&ready[sl], q + 1, abort_flag, spin_limit, status);
+      if (!landed) {  // the hand-off gave up: this field from global memory, per lane
+        per_lane<float, MATH, POST>(it, F, f0 + q, f0 + q + 1, x, ok, valid, cw, o);
+        continue;
+      }
       const float* base = (const float*)F.sc(f0 + q);
       const int sh = (int)((((unsigned long long)base >> 2) + (unsigned)(b.x0 - 1)) & 3);
       if (ok) {
@@ -589,7 +610,7 @@ __global__ __launch_bounds__(kPCThreads) void k_plan_pc(PlanArgs A) {
     int f0 = (int)grp * A.g, f1 = min(it.nf, f0 + A.g);
     ItemFields F{it.src, it.dst, it.lin, it.mn, it.mx};
     pc_tile<MATH, POST, TP>(it, F, ptile, f0, f1, (lds_float*)ring, (lds_flag*)ready, (lds_flag*)freed, red,
-                            (lds_flag*)&abort_flag, A.level_inner);
+                            (lds_flag*)&abort_flag, A.level_inner, A.spin_limit, A.status);
   }
 }
 
@@ -836,9 +857,9 @@ int adz_plan_create(const adz_item* items, int nitems, int fdtype, int math, int
     dv[q].mx = (void* const*)(intptr_t)(4 * nptr_total + pofs);
     pofs += a.nptr;
   }
-  size_t ib = sizeof(DevItem) * nitems, pb = sizeof(void*) * ptrs.size();
+  size_t ib = sizeof(DevItem) * nitems, pb = sizeof(void*) * ptrs.size(), sb = 64;  // sb: the status word
   void* mem = nullptr;
-  if (hipMalloc(&mem, ib + pb) != hipSuccess) return fail(ADZ_ENOMEM, "adz_plan_create: hipMalloc");
+  if (hipMalloc(&mem, ib + pb + sb) != hipSuccess) return fail(ADZ_ENOMEM, "adz_plan_create: hipMalloc");
   void** dptr = (void**)((char*)mem + ib);
   for (auto& d : dv) {
     d.src = (const void* const*)(dptr + (intptr_t)d.src);
@@ -847,15 +868,17 @@ int adz_plan_create(const adz_item* items, int nitems, int fdtype, int math, int
     d.mn = (void* const*)(dptr + (intptr_t)d.mn);
     d.mx = (void* const*)(dptr + (intptr_t)d.mx);
   }
+  int* status = (int*)((char*)mem + ib + pb);
   if (hipMemcpy(mem, dv.data(), ib, hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(dptr, ptrs.data(), pb, hipMemcpyHostToDevice) != hipSuccess) {
+      hipMemcpy(dptr, ptrs.data(), pb, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemset(status, 0, sb) != hipSuccess) {
     (void)hipFree(mem);
     return fail(ADZ_EHIP, "adz_plan_create: upload");
   }
   adz_plan* p = new adz_plan;
   p->fdtype = fdtype; p->pdtype = pdt; p->math = math; p->post = post; p->tune = T; p->dmem = mem;
   p->args = PlanArgs{(const DevItem*)mem, nitems, tiles, T.fields_per_pass, T.points_outer, T.xcd_chunks, T.stage,
-                     T.level_inner};
+                     T.level_inner, 1 << 20, status};
   long grid = (long)T.blocks_per_cu * device_cus();
   if (grid > tiles) grid = tiles;
   if (T.xcd_chunks) grid = std::max(8L, (grid / 8) * 8);
@@ -871,6 +894,21 @@ int adz_plan_execute(const adz_plan* p, adz_stream stream) {
   void* args[] = {&a};
   const unsigned nthr = (p->tune.stage == 3 && p->fdtype == ADZ_F32) ? kPCThreads : kTile;
   ADZ_HIP_CHECK(hipLaunchKernel(kern, dim3(p->grid), dim3(nthr), args, 0, (hipStream_t)stream));
+  return ADZ_OK;
+}
+
+int adz_plan_status(adz_plan* p, int* aborted_waits) {
+  if (!p || !aborted_waits) return fail(ADZ_EINVAL, "adz_plan_status: null argument");
+  int v = 0;
+  ADZ_HIP_CHECK(hipMemcpy(&v, p->args.status, sizeof(int), hipMemcpyDeviceToHost));
+  if (v) ADZ_HIP_CHECK(hipMemset(p->args.status, 0, sizeof(int)));
+  *aborted_waits = v;
+  return ADZ_OK;
+}
+
+int adz_plan_set_spin_limit(adz_plan* p, int polls) {
+  if (!p || polls < 0) return fail(ADZ_EINVAL, "adz_plan_set_spin_limit: bad argument");
+  p->args.spin_limit = polls;
   return ADZ_OK;
 }
 
@@ -905,6 +943,10 @@ int adz_plan_autotune(const adz_item* items, int nitems, int fdtype, int math, i
                          hipEventElapsedTime(ms, e0, e1) != hipSuccess))
       rc = fail(ADZ_EHIP, "adz_plan_autotune: timing");
     if (rc == ADZ_OK) *ms /= reps;
+    // a candidate whose staging waits gave up timed a fallback, not itself: drop it
+    int aborted = 0;
+    if (rc == ADZ_OK) rc = adz_plan_status(p, &aborted);
+    if (rc == ADZ_OK && aborted) *ms = 3.0e38f;
     int rf = adz_plan_free(p);
     return rc != ADZ_OK ? rc : rf;
   };

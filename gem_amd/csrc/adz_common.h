@@ -21,6 +21,16 @@ int fail(int code, const std::string& msg);
     if (e_ != hipSuccess) return ::adz::fail(ADZ_EHIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
   } while (0)
 
+// ------------------------------------------------------------------ context tables
+// Air-mass tables of a context (adz_set): Ver_dz_8%t(k) and Ver_idz_8%t(k) indexed by k
+// (device), the tile size and Adz_k0t.  False for a null context.
+struct AirmassTables {
+  const double* dz;
+  const double* idz;
+  int nk, ni, nj, k0t;
+};
+bool ctx_airmass(const adz_ctx* c, AirmassTables* t);
+
 // ------------------------------------------------------------------ storage types
 typedef __hip_bfloat16 bf16;
 

@@ -320,6 +320,7 @@ struct adz_ctx {
   adz_ztab* zth = nullptr;
   void* tables = nullptr;  // device copy of the tables
   void* qvtab = nullptr;   // delz_t and the cubic / quintic denominators (BICHQV)
+  void* amtab = nullptr;   // Ver_dz_8%t(k), Ver_idz_8%t(k), k = 0..nk (air mass)
   // device state
   void* uvw_d = nullptr; void* warr = nullptr; void* uvw_dep = nullptr; void* pxyzm = nullptr;
   double* wpxyz = nullptr;
@@ -357,7 +358,7 @@ double lag3(double zz, double z1, double z2, double z3, double z4) {
 unsigned nb(long n) { return (unsigned)((n + 255) / 256); }
 
 void free_ctx(adz_ctx* c) {
-  void* bufs[] = {c->tables, c->qvtab, c->uvw_d, c->warr, c->uvw_dep, c->pxyzm, c->wpxyz,
+  void* bufs[] = {c->tables, c->qvtab, c->amtab, c->uvw_d, c->warr, c->uvw_dep, c->pxyzm, c->wpxyz,
                   c->pm, c->pmu, c->pmv, c->pt, c->pb};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
@@ -536,6 +537,21 @@ int adz_set(const adz_grid* g, int pos_dtype, int itraj, adz_ctx** out) {
     }
   }
 
+  // layer thicknesses of the air mass (set_zeta.F90:271-275): Ver_dz_8%t(k) =
+  // Ver_z_8%m(k+1) - Ver_z_8%m(k) and its inverse, k = 1..nk (entry 0 unused)
+  {
+    std::vector<double> a(2 * (size_t)(nk + 1), 0.0);
+    for (int k = 1; k <= nk; k++) {
+      a[k] = zm[k + 1] - zm[k];
+      a[(nk + 1) + k] = 1.0 / a[k];
+    }
+    if (hipMalloc(&c->amtab, 8 * a.size()) != hipSuccess) { free_ctx(c); return fail(ADZ_ENOMEM, "adz_set: tables"); }
+    if (hipMemcpy(c->amtab, a.data(), 8 * a.size(), hipMemcpyHostToDevice) != hipSuccess) {
+      free_ctx(c);
+      return fail(ADZ_EHIP, "adz_set: table upload");
+    }
+  }
+
   // vertical descriptors (adz_set.F90:128-131) over the extended arrays
   int st = adz_vsearch_setup_plus(&zm[1], nk, P.nit, P.njt, 1 - g->i0g, 1 - g->j0g, &c->zq);
   if (st == ADZ_OK) st = adz_vsearch_setup_plus(&zt[1], nk, P.nit, P.njt, 1 - g->i0g, 1 - g->j0g, &c->zth);
@@ -622,6 +638,15 @@ bool adz::ctx_qv(const adz_ctx* c, QvDev* q) {
   q->lminx = c->P.lminx; q->lmaxx = c->P.lminx + c->P.nit - 1;
   q->lminy = c->P.lminy; q->lmaxy = c->P.lminy + c->P.njt - 1;
   q->pos_dtype = c->pos_dtype;
+  return true;
+}
+
+bool adz::ctx_airmass(const adz_ctx* c, AirmassTables* t) {
+  if (!c || !c->amtab) return false;
+  const int nk = c->P.nk;
+  t->dz = (const double*)c->amtab;
+  t->idz = t->dz + (nk + 1);
+  t->nk = nk; t->ni = c->P.ni; t->nj = c->P.nj; t->k0t = c->P.k0t;
   return true;
 }
 

@@ -10,6 +10,8 @@
 !         154-163 runs it: interpolation on Adz_pb, the piloting values of the
 !         destinations saved before and restored after the fixer)
 !        adz_f_driver sizes      (prints the byte sizes of the bind(C) types)
+!        adz_f_driver refuse_f64 ilmc|qv   (FP64 fields given to ILMC / the quintic path:
+!                                 must stop with a message before any device call)
 ! The input is a flat little-endian stream (written by the caller):
 !   int32  ni nj nk halox haloy pil itraj nfld mh pos_dtype
 !   real64 inv_hx inv_hy inv_rayt zmin zmax wmstar_nk idz_t_nkm1 dt
@@ -47,6 +49,18 @@ program adz_f_driver
    call get_command_argument(3, mode)
    if (trim(fin) == 'sizes') then
       call print_sizes()
+      stop
+   end if
+   if (trim(fin) == 'refuse_f64') then  ! FP64 fields with a kernel that reads `real`: must stop
+      Adz_field_dtype = ADZ_F64
+      if (trim(fout) == 'qv') then
+         allocate (Adz_stack(1))
+         call adz_tricub_hlt(Adz_stack, 1, c_null_ptr, c_null_ptr, 1, 1, 1, 1, 1, 1, F_ext_L=.false., F_QV_L=.true.)
+      else
+         allocate (Adz_stack(1), Adz_post(1))
+         call ILMC_LAM(1, 'TR/X:M', 1, 1, 1, 1)
+      end if
+      write (*, '(a)') 'not refused'
       stop
    end if
    open (newunit=u, file=trim(fin), access='stream', form='unformatted', status='old')

@@ -192,6 +192,10 @@ contains
          write (error_unit, *) 'Source must be NOT expended when using quintic - abort'
          error stop 1
       end if
+      if (qv_L .and. Adz_field_dtype /= ADZ_F32) then  ! the quintic kernel reads `real` fields
+         write (error_unit, *) 'adz_tricub_hlt: F_QV_L needs FP32 fields (Adz_field_dtype = ADZ_F32)'
+         error stop 1
+      end if
       if ((F_in - F_i0 + 1) <= 0 .or. (F_jn - F_j0 + 1) <= 0) return
       ! adz_interp_hlt.F90:91-96: the LAM boundary flux of the fixer's tracers
       if (.not. ext_L .and. present(F_post)) then
@@ -385,6 +389,10 @@ contains
       type(adz_ilmc_tracer) :: tr(1)
       type(adz_ilmc_domain) :: dom
       type(c_ptr) :: h
+      if (Adz_field_dtype /= ADZ_F32) then  ! the ILMC kernels read and write `real` arrays
+         write (error_unit, *) 'ILMC_LAM: needs FP32 fields (Adz_field_dtype = ADZ_F32) for ', trim(F_name_S)
+         error stop 1
+      end if
       if (.not. c_associated(airm0)) then
          write (error_unit, *) 'ILMC_LAM: airm0 must be set before ', trim(F_name_S)
          error stop 1

@@ -252,6 +252,15 @@ int adz_plan_create(const adz_item* items, int nitems, int fdtype, int math, int
                     adz_plan** out);
 int adz_plan_execute(const adz_plan* p, adz_stream stream);
 int adz_plan_tune(const adz_plan* p, adz_tune* out); /* the parameters in use */
+/* Staging waits of the plan's launches (stage 3) that gave up since the last call, read
+ * from a device counter and reset; the fields they concern were evaluated from global
+ * memory by per-lane loads, so the results stay exact.  Synchronises the device.  No
+ * reference counterpart. */
+int adz_plan_status(adz_plan* p, int* aborted_waits);
+/* Polls of one bounded staging wait before it gives up (default 2^20); 0 makes every wait
+ * that is not already satisfied give up at once (a test of the fallback).  No reference
+ * counterpart. */
+int adz_plan_set_spin_limit(adz_plan* p, int polls);
 int adz_plan_free(adz_plan* p);
 /* Project autotuner for the persistent kernel's tile parameters: a coordinate search
  * (staging, tile order, fields per pass, blocks per CU, loop order, XCD chunking), each candidate
@@ -408,6 +417,38 @@ int adz_ilmc_create(const adz_ilmc_tracer* tr, int ntr, const adz_ilmc_domain* d
  * ni + (2S+1) nj + (2S+1)^2 nk launches). */
 int adz_ilmc_lam(adz_ilmc* il, const void* air_m, long long* counts, adz_stream stream);
 int adz_ilmc_free(adz_ilmc* il);
+
+/* ------------------------------------------------------------------------------------
+ * Level 4d: the fixer's air masses.  get_air_mass_hlt (get_air_mass_hlt.F90:18-103) at
+ * time P (F_time = 1) and time M (F_time = 0), and pr_t/pr_s at time M, as set_post_tr_hlt
+ * forms them before the tracers' post-processing (set_post_tr_hlt.F90:44-70).
+ * ----------------------------------------------------------------------------------*/
+typedef struct adz_airmass_fields {
+  int minx, maxx, miny, maxy;  /* layout of every array: (l_minx:l_maxx, l_miny:l_maxy, .)   */
+  const void* pm_p;            /* pw_pm_plus  (nk+1 levels): momentum-level pressure at P     */
+  const void* pm_m;            /* pw_pm_moins (nk+1 levels), at M                            */
+  const void* hu_p;            /* tracers_P(Tr3d_hu): read when dry_mixing_ratio             */
+  const void* hu_m;            /* tracers_M(Tr3d_hu)                                          */
+  const void* const* hyd_p;    /* HOST array of nhyd DEVICE pointers: the Tr3d_wload tracers
+                                  of trt1 in tracer order; read when dry_mixing_ratio and
+                                  wload (sumhydro_hlt.F90:45-60)                              */
+  const void* const* hyd_m;    /* the same of trt0                                            */
+  int nhyd;                    /* 0..8                                                        */
+  const void* pt_m;            /* pw_pt_moins (nk levels)                                     */
+  const void* p0_m;            /* pw_p0_moins (2-D)                                           */
+  const double* area;          /* geomh_area_8 (2-D, FP64)                                    */
+  int dry_mixing_ratio;        /* Schm_dry_mixing_ratio_L                                     */
+  int wload;                   /* Schm_wload_L                                                */
+} adz_airmass_fields;
+/* airm1 (time P), airm0 (time M) and pkps (pr_t/pr_s at M, may be NULL) of storage type
+ * dtype (ADZ_F32 = the reference's `real`, or ADZ_F64; every input array has that type
+ * except area), DEVICE pointers to element (1,1,1) of the layout.  The air masses are
+ * written on the whole layout (zero outside 1..l_ni x 1..l_nj and below Adz_k0t, as the
+ * reference does); pkps on 1..l_ni x 1..l_nj x 1..nk.  The layer thicknesses
+ * Ver_dz_8%t / Ver_idz_8%t come from the context's levels (set_zeta.F90:271-275);
+ * grav is tdpack's grav_8.  No host synchronisation: capturable. */
+int adz_set_post_tr(const adz_ctx* c, const adz_airmass_fields* f, double grav, void* airm1, void* airm0, void* pkps,
+                    int dtype, adz_stream stream);
 
 #ifdef __cplusplus
 }

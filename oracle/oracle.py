@@ -597,3 +597,49 @@ class RefIlmc:
         moved = np.zeros(1, np.int64)
         self.f(_p(f), _p(mn), _p(mx), _p(air), _p(_ilmc_dom(layout, box, sweep_max, 0, autobar)), _p(moved))
         return int(moved[0])
+
+
+# ------------------------------------------------------------------ air mass (fixer inputs)
+def air_mass(pm, hu, hyd, area, zm, layout, tile, k0, dry, wload, grav):
+    """get_air_mass_hlt (get_air_mass_hlt.F90:18-103) for one time level, on host arrays of
+    the model layout (minx:maxx, miny:maxy, .) as (k, j, i) numpy arrays: pm has nk+1 levels
+    (pw_pm_*), hu the humidity, hyd the list of hydrometeor tracers in tracer order, area
+    the FP64 geomh_area_8 (j, i); zm = Ver_z_8%m(0:nk+1).  Arithmetic as the reference writes
+    it: sumq in FP64 from 0, each hydrometeor then the humidity (sumhydro_hlt.F90:35-60,
+    :67-79, only with Schm_dry_mixing_ratio_L, hydrometeors also needing Schm_wload_L); the
+    pressure difference in the storage type; dens = diff*(1-sumq)*idz*(1/g) and
+    air = dens*area*dz left to right in FP64 (:83-94), Ver_dz_8%t(k) = zm(k+1) - zm(k) and
+    Ver_idz_8%t = 1/dz (set_zeta.F90:271-275); zero outside the tile and below k0 (:84-91)."""
+    minx, maxx, miny, maxy, nk = layout
+    ni, nj = tile
+    dt = pm.dtype
+    out = np.zeros((nk, maxy - miny + 1, maxx - minx + 1), dt)
+    zm = np.asarray(zm, np.float64)
+    inv_grav = 1.0 / grav
+    sj = slice(1 - miny, nj - miny + 1)
+    si = slice(1 - minx, ni - minx + 1)
+    for k in range(k0, nk + 1):
+        sumq = np.zeros((nj, ni))
+        if dry and wload:
+            for h in hyd:
+                sumq = sumq + h[k - 1, sj, si].astype(np.float64)
+        if dry:
+            sumq = sumq + hu[k - 1, sj, si].astype(np.float64)
+        diff = (pm[k, sj, si] - pm[k - 1, sj, si]).astype(np.float64)   # in the storage type
+        dz = zm[k + 1] - zm[k]
+        idz = 1.0 / dz
+        dens = diff * (1.0 - sumq) * idz * inv_grav
+        out[k - 1, sj, si] = (dens * area[sj, si] * dz).astype(dt)
+    return out
+
+
+def pkps(pt, p0, layout, tile):
+    """set_post_tr_hlt.F90:62-70: pr_t / p0 at time M on 1..l_ni x 1..l_nj x 1..nk, in the
+    storage type (IEEE division); elsewhere the array keeps what it held (zeros here)."""
+    minx, maxx, miny, maxy, nk = layout
+    ni, nj = tile
+    out = np.zeros_like(pt)
+    sj = slice(1 - miny, nj - miny + 1)
+    si = slice(1 - minx, ni - minx + 1)
+    out[:, sj, si] = pt[:, sj, si] / p0[None, sj, si]
+    return out

@@ -21,3 +21,15 @@ def test_bind_c_type_sizes_match_c_abi():
     sizes = [int(x) for x in line.split()[1:]]
     assert sizes == [C.sizeof(s) for s in (A.AdzGrid, A.AdzField, A.AdzItem, A.AdzTune, A.AdzBcTracer,
                                            A.AdzBcDomain, A.AdzFluxTracer, A.AdzFluxDomain)]
+
+
+@pytest.mark.parametrize("what", ["ilmc", "qv"])
+def test_fp64_fields_refused_where_the_kernel_reads_real(what):
+    """ILMC_LAM and adz_tricub_hlt(F_QV_L) run kernels on `real` arrays: with FP64 field
+    storage the Fortran layer stops with a message before any device call (here: no GPU)."""
+    if not os.path.exists(DRIVER):
+        pytest.skip("Fortran driver not built (no amdflang)")
+    out = subprocess.run([DRIVER, "refuse_f64", what], capture_output=True, text=True, timeout=60)
+    assert out.returncode != 0
+    assert "FP32 fields" in out.stderr + out.stdout
+    assert "not refused" not in out.stdout

@@ -153,24 +153,7 @@ def test_hlt_f32_math_error_vs_fp64_oracle(T):
     ctx.free()
 
 
-@pytest.mark.parametrize("tune", [None, dict(fields_per_pass=1, points_outer=0, blocks_per_cu=2, xcd_chunks=0),
-                                  dict(fields_per_pass=3, points_outer=1, blocks_per_cu=4, xcd_chunks=1),
-                                  dict(fields_per_pass=5, points_outer=0, blocks_per_cu=4, xcd_chunks=1, stage=1),
-                                  dict(fields_per_pass=2, points_outer=1, blocks_per_cu=8, xcd_chunks=0, stage=1),
-                                  dict(fields_per_pass=4, points_outer=1, blocks_per_cu=4, xcd_chunks=1, stage=2,
-                                       level_inner=1),
-                                  dict(fields_per_pass=3, points_outer=0, blocks_per_cu=4, xcd_chunks=1, stage=1,
-                                       level_inner=1),
-                                  dict(fields_per_pass=5, points_outer=0, blocks_per_cu=3, xcd_chunks=1, stage=3),
-                                  dict(fields_per_pass=2, points_outer=1, blocks_per_cu=2, xcd_chunks=0, stage=3,
-                                       level_inner=1),
-                                  dict(fields_per_pass=4, points_outer=0, blocks_per_cu=4, xcd_chunks=1, stage=2,
-                                       level_inner=4),
-                                  dict(fields_per_pass=5, points_outer=1, blocks_per_cu=2, xcd_chunks=0, stage=1,
-                                       level_inner=3),
-                                  dict(fields_per_pass=3, points_outer=0, blocks_per_cu=3, xcd_chunks=1, stage=3,
-                                       level_inner=2)])
-def test_plan_matches_single_calls(T, tune):
+def _plan_case(T):
     torch, L, A = T
     cfgs = [S.Config(f"p{s}", 40 + 16 * s, 36 + 8 * s, 16, 4, 300.0, jets=True, cmax_xy=2.0, seed=60 + s)
             for s in range(3)]
@@ -193,10 +176,67 @@ def test_plan_matches_single_calls(T, tune):
         items.append(dict(geom=ctx.ztab(True), xyz=ctx.array(A.ADZ_A_PT), box=box, dlayout=dl,
                           fields=A.field_list(stack, src_off, dst_off)))
         refs.append((stack, ref))
+    return items, refs, keep
+
+
+@pytest.mark.parametrize("tune", [None, dict(fields_per_pass=1, points_outer=0, blocks_per_cu=2, xcd_chunks=0),
+                                  dict(fields_per_pass=3, points_outer=1, blocks_per_cu=4, xcd_chunks=1),
+                                  dict(fields_per_pass=5, points_outer=0, blocks_per_cu=4, xcd_chunks=1, stage=1),
+                                  dict(fields_per_pass=2, points_outer=1, blocks_per_cu=8, xcd_chunks=0, stage=1),
+                                  dict(fields_per_pass=4, points_outer=1, blocks_per_cu=4, xcd_chunks=1, stage=2,
+                                       level_inner=1),
+                                  dict(fields_per_pass=3, points_outer=0, blocks_per_cu=4, xcd_chunks=1, stage=1,
+                                       level_inner=1),
+                                  dict(fields_per_pass=5, points_outer=0, blocks_per_cu=3, xcd_chunks=1, stage=3),
+                                  dict(fields_per_pass=2, points_outer=1, blocks_per_cu=2, xcd_chunks=0, stage=3,
+                                       level_inner=1),
+                                  dict(fields_per_pass=4, points_outer=0, blocks_per_cu=4, xcd_chunks=1, stage=2,
+                                       level_inner=4),
+                                  dict(fields_per_pass=5, points_outer=1, blocks_per_cu=2, xcd_chunks=0, stage=1,
+                                       level_inner=3),
+                                  dict(fields_per_pass=3, points_outer=0, blocks_per_cu=3, xcd_chunks=1, stage=3,
+                                       level_inner=2)])
+def test_plan_matches_single_calls(T, tune):
+    torch, L, A = T
+    items, refs, keep = _plan_case(T)
     plan = A.Plan(items, L.ADZ_F32, math=L.ADZ_MATH_F32, tune=tune)
     plan.execute()
     plan.execute()  # idempotent
     torch.cuda.synchronize()
+    for stack, ref in refs:
+        for a, b in zip(stack, ref):
+            assert torch.equal(a["dst"], b["dst"])
+    assert plan.status() == 0   # no staging wait gave up
+    plan.free()
+    for c in keep:
+        c.free()
+
+
+def test_stage3_waits_that_give_up_fall_back_exactly(T):
+    """The staging hand-off of mode 3 forced to give up from the host (a spin bound of 0
+    polls, so every wait that is not already satisfied gives up at once; no kernel is
+    made to hang): the plan still equals the single calls bit for bit (the fields whose
+    slots were not published are read from global memory), and the plan's status word
+    counts the waits that gave up.  With the default bound, a later run reports none."""
+    torch, L, A = T
+    items, refs, keep = _plan_case(T)
+    plan = A.Plan(items, L.ADZ_F32, math=L.ADZ_MATH_F32,
+                  tune=dict(fields_per_pass=5, points_outer=0, blocks_per_cu=3, xcd_chunks=1, stage=3))
+    plan.set_spin_limit(0)
+    for stack, _ in refs:
+        for a in stack:
+            a["dst"].zero_()
+    plan.execute()
+    torch.cuda.synchronize()
+    for stack, ref in refs:
+        for a, b in zip(stack, ref):
+            assert torch.equal(a["dst"], b["dst"])
+    assert plan.status() > 0
+    assert plan.status() == 0   # read and reset
+    plan.set_spin_limit(1 << 20)
+    plan.execute()
+    torch.cuda.synchronize()
+    assert plan.status() == 0
     for stack, ref in refs:
         for a, b in zip(stack, ref):
             assert torch.equal(a["dst"], b["dst"])
