@@ -166,7 +166,7 @@ template <class S>
 static void traj_run(const Grid& g, int init, double dt, int advec, double bA_m, double bA, int niter, int order,
                      const float* uu, const float* vv, const float* ww, const float* ut0, const float* vt0,
                      const float* zdt0, S* pxyzm, double* wpxyz, S* uvw_dep, S* pm, S* pmu, S* pmv, S* pt, S* pb,
-                     S* uvw_d, S* warr) {
+                     S* uvw_d, S* warr, int klo, int khi) {
   TrajState<S> s;
   alloc_state<S>(g, &s);
   if (init) inittraj<S>(g, &s);
@@ -174,7 +174,7 @@ static void traj_run(const Grid& g, int init, double dt, int advec, double bA_m,
     std::memcpy(s.pxyzm.data(), pxyzm, s.pxyzm.size() * sizeof(S));
     std::memcpy(s.wpxyz.data(), wpxyz, s.wpxyz.size() * 8);
   }
-  traject<S>(g, dt, advec, bA_m, bA, niter, order, uu, vv, ww, ut0, vt0, zdt0, &s);
+  traject<S>(g, dt, advec, bA_m, bA, niter, order, uu, vv, ww, ut0, vt0, zdt0, &s, klo, khi);
   auto out = [](auto* dst, const auto& v) { if (dst) std::memcpy(dst, v.data(), v.size() * sizeof(v[0])); };
   out(pxyzm, s.pxyzm); out(wpxyz, s.wpxyz); out(uvw_dep, s.uvw_dep); out(pm, s.pm); out(pmu, s.pmu);
   out(pmv, s.pmv); out(pt, s.pt); out(pb, s.pb); out(uvw_d, s.uvw_d);
@@ -188,16 +188,16 @@ static void traj_run(const Grid& g, int init, double dt, int advec, double bA_m,
 extern "C" void ora_traject(const void* h, int smode, int init, double dt, int advec, double bA_m, double bA, int niter,
                  int order, const float* uu, const float* vv, const float* ww, const float* ut0, const float* vt0,
                  const float* zdt0, void* pxyzm, double* wpxyz, void* uvw_dep, void* pm, void* pmu, void* pmv,
-                 void* pt, void* pb, void* uvw_d, void* warr) {
+                 void* pt, void* pb, void* uvw_d, void* warr, int klo, int khi) {
   const Grid& g = *(const Grid*)h;
   if (smode == 1)
     traj_run<float>(g, init, dt, advec, bA_m, bA, niter, order, uu, vv, ww, ut0, vt0, zdt0, (float*)pxyzm, wpxyz,
                     (float*)uvw_dep, (float*)pm, (float*)pmu, (float*)pmv, (float*)pt, (float*)pb, (float*)uvw_d,
-                    (float*)warr);
+                    (float*)warr, klo, khi);
   else
     traj_run<double>(g, init, dt, advec, bA_m, bA, niter, order, uu, vv, ww, ut0, vt0, zdt0, (double*)pxyzm, wpxyz,
                      (double*)uvw_dep, (double*)pm, (double*)pmu, (double*)pmv, (double*)pt, (double*)pb,
-                     (double*)uvw_d, (double*)warr);
+                     (double*)uvw_d, (double*)warr, klo, khi);
 }
 
 // -------------------------------- Bermejo-Conde -------------------------------------

@@ -50,7 +50,7 @@ def lib():
         L.ora_grid_table.argtypes = [vp, i, vp]
         L.ora_level_search.argtypes = [vp, i, l, vp, vp]
         L.ora_zindx.argtypes = [vp, l, vp, vp]
-        L.ora_traject.argtypes = [vp, i, i, d, i, d, d, i, i] + [vp] * 6 + [vp] * 10
+        L.ora_traject.argtypes = [vp, i, i, d, i, d, d, i, i] + [vp] * 6 + [vp] * 10 + [i, i]
         L.ora_almost_zero.restype = i
         L.ora_almost_zero.argtypes = [d]
         L.ora_fsum.restype = d
@@ -288,26 +288,33 @@ class Grid:
                     pb=3 * (i["inb"] - i["i0b"] + 1) * (i["jnb"] - i["j0b"] + 1) * nk,
                     uvw_d=3 * i["nij"] * nk, arr=3 * nxy * nk)
 
-    def traject(self, winds, dt, storage="f32", state=None, niter=3, advec=1, bA_m=0.6, bA=0.6, order=ORD_AVX2):
+    def traject(self, winds, dt, storage="f32", state=None, niter=3, advec=1, bA_m=0.6, bA=0.6, order=ORD_AVX2,
+                band=None, want=None):
         """adz_traject restatement.  storage 'f32' = reference `real` storage, 'f64' = FP64.
         state: dict with 'pxyzm' and 'wpxyz' carried from a previous step (None: inittraj,
-        and then max(6, niter) iterations as adz_inittraj sets)."""
+        and then max(6, niter) iterations as adz_inittraj sets).  band = (klo, khi): only
+        the position sets of those levels are produced (exactly: the departure points of
+        levels klo-1..khi+2 are iterated); want: names of the outputs to return (default
+        all), to spare host memory at large sizes."""
         dt_ = np.float32 if storage == "f32" else np.float64
         sz = self.sizes()
-        out = {k: np.zeros(sz[s], dt_) for k, s in (("uvw_dep", "p3"), ("pm", "pm"), ("pmu", "pmu"), ("pmv", "pmv"),
-                                                    ("pt", "pt"), ("pb", "pb"), ("uvw_d", "uvw_d"), ("warr", "arr"))}
+        names = (("uvw_dep", "p3"), ("pm", "pm"), ("pmu", "pmu"), ("pmv", "pmv"), ("pt", "pt"), ("pb", "pb"),
+                 ("uvw_d", "uvw_d"), ("warr", "arr"))
+        out = {k: (np.zeros(sz[s], dt_) if want is None or k in want else None) for k, s in names}
         init = state is None
         pxyzm = np.zeros(sz["p3"], dt_) if init else np.ascontiguousarray(state["pxyzm"], dtype=dt_).copy()
         wpxyz = np.zeros(sz["wp"]) if init else np.ascontiguousarray(state["wpxyz"], dtype=np.float64).copy()
         nit = max(6, niter) if init else niter
         w = {k: np.ascontiguousarray(np.asarray(winds[k]), dtype=np.float32)
              for k in ("uu_ext", "vv_ext", "ww_ext", "ut0", "vt0", "zdt0")}
+        klo, khi = band if band is not None else (1, self.g["nk"])
         lib().ora_traject(self.h, 1 if storage == "f32" else 0, 1 if init else 0, dt, advec, bA_m, bA, nit, order,
                           *[_p(w[k]) for k in ("uu_ext", "vv_ext", "ww_ext", "ut0", "vt0", "zdt0")],
-                          _p(pxyzm), _p(wpxyz), *[_p(out[k]) for k in ("uvw_dep", "pm", "pmu", "pmv", "pt", "pb",
-                                                                       "uvw_d", "warr")])
+                          _p(pxyzm), _p(wpxyz), *[None if out[k] is None else _p(out[k])
+                                                  for k in ("uvw_dep", "pm", "pmu", "pmv", "pt", "pb", "uvw_d",
+                                                            "warr")], int(klo), int(khi))
         out["pxyzm"], out["wpxyz"] = pxyzm, wpxyz
-        return out
+        return {k: v for k, v in out.items() if v is not None}
 
     def __del__(self):
         if getattr(self, "h", None) and _lib is not None:

@@ -354,9 +354,13 @@ inline void traj_point(const Grid& g, const Dt& t, int i, int j, int k, int orde
 inline double clampd(double v, double lo, double hi) { return std::min(std::max(v, lo), hi); }
 
 // adz_interp_traj.F90:16-187 (Adz_slt_winds = .false.)
+// klo..khi: the levels whose position sets are produced (1..nk: all); the departure points
+// of levels klo-1..khi+2 must be current (the thermodynamic sets read levels k-1..k+2).
 template <class S>
-inline void interp_traj(const Grid& g, const Dt& t, double dt, const float* zdt0, TrajState<S>* s) {
+inline void interp_traj(const Grid& g, const Dt& t, double dt, const float* zdt0, TrajState<S>* s, int klo = 1,
+                        int khi = 1 << 30) {
   (void)dt;
+  auto band = [&](int k) { return k >= klo && k <= khi; };
   const int nk = g.nk;
   const double half = 0.5, aa = -0.0625, bb = 0.5625;
   const double zmin_b = 0.0, zmax_b = (double)(nk + 1);
@@ -368,7 +372,7 @@ inline void interp_traj(const Grid& g, const Dt& t, double dt, const float* zdt0
   Ext3 e{g.lminx, g.lminy, g.nit, g.njt};
   auto W = [&](int i, int j, int k, int c) { return s->wpxyz[wp_idx(g, i, j, k, c)]; };
   auto UD3 = [&](int i, int j, int k) { return (double)s->uvw_dep[p3_idx(g.ni, g.nj, g.k0m, i, j, k) + 2]; };
-  for (int k = g.k0; k <= nk; k++)
+  for (int k = std::max(g.k0, klo); k <= std::min(nk, khi); k++)
     for (int j = g.j0; j <= g.jn; j++)
       for (int i = g.i0u; i <= g.inu; i++) {
         double xt = aa * (W(i - 1, j, k, 1) + W(i + 2, j, k, 1)) + bb * (W(i, j, k, 1) + W(i + 1, j, k, 1)) - 0.5;
@@ -379,7 +383,7 @@ inline void interp_traj(const Grid& g, const Dt& t, double dt, const float* zdt0
         s->pmu[q + 1] = (S)clampd(yt, g.iminposy, g.imaxposy);
         s->pmu[q + 2] = (S)clampd(zt, zmin_b, zmax_b);
       }
-  for (int k = g.k0; k <= nk; k++)
+  for (int k = std::max(g.k0, klo); k <= std::min(nk, khi); k++)
     for (int j = g.j0v; j <= g.jnv; j++)
       for (int i = g.i0; i <= g.in; i++) {
         double xt = aa * (W(i, j - 1, k, 1) + W(i, j + 2, k, 1)) + bb * (W(i, j, k, 1) + W(i, j + 1, k, 1));
@@ -392,7 +396,7 @@ inline void interp_traj(const Grid& g, const Dt& t, double dt, const float* zdt0
       }
   auto PB = [&](int i, int j, int k) { return box_idx(g.i0b, g.inb, g.j0b, g.jnb, 1, i, j, k); };
   auto Z0 = [&](int i, int j, int k) { return (double)zdt0[e.idx(i, j, k)]; };
-  for (int k = std::max(k00, 2); k <= nk - 2; k++)
+  for (int k = std::max(std::max(k00, 2), klo); k <= std::min(nk - 2, khi); k++)
     for (int j = g.j0b; j <= g.jnb; j++)
       for (int i = g.i0b; i <= g.inb; i++) {
         double xt = g.vw1t[k] * W(i, j, k - 1, 1) + g.vw2t[k] * W(i, j, k, 1) + g.vw3t[k] * W(i, j, k + 1, 1) +
@@ -407,7 +411,7 @@ inline void interp_traj(const Grid& g, const Dt& t, double dt, const float* zdt0
         s->pb[q + 1] = (S)clampd(yt, g.iminposy, g.imaxposy);
         s->pb[q + 2] = (S)zindx(g, zt);
       }
-  {
+  if (band(nk - 1)) {
     int k = nk - 1;
     for (int j = g.j0b; j <= g.jnb; j++)
       for (int i = g.i0b; i <= g.inb; i++) {
@@ -421,7 +425,7 @@ inline void interp_traj(const Grid& g, const Dt& t, double dt, const float* zdt0
         s->pb[q + 2] = (S)zindx(g, zt);
       }
   }
-  {
+  if (band(nk)) {
     int k = nk;
     for (int j = g.j0b; j <= g.jnb; j++)
       for (int i = g.i0b; i <= g.inb; i++) {
@@ -438,7 +442,7 @@ inline void interp_traj(const Grid& g, const Dt& t, double dt, const float* zdt0
         s->pb[PB(i, j, k) + 2] = (S)zindx(g, zt);
       }
   }
-  if (k00 == 1) {
+  if (k00 == 1 && band(1)) {
     for (int j = g.j0b; j <= g.jnb; j++)
       for (int i = g.i0b; i <= g.inb; i++) {
         double xt = (W(i, j, 1, 1) + W(i, j, 2, 1)) * half;
@@ -451,7 +455,7 @@ inline void interp_traj(const Grid& g, const Dt& t, double dt, const float* zdt0
         s->pb[q + 2] = (S)zindx(g, zt);
       }
   }
-  for (int k = g.k0t; k <= nk; k++)
+  for (int k = std::max(g.k0t, klo); k <= std::min(nk, khi); k++)
     for (int j = g.j0; j <= g.jn; j++)
       for (int i = g.i0; i <= g.in; i++) {
         long q = box_idx(g.i0, g.in, g.j0, g.jn, g.k0t, i, j, k), r = PB(i, j, k);
@@ -461,29 +465,33 @@ inline void interp_traj(const Grid& g, const Dt& t, double dt, const float* zdt0
 
 // adz_traject.F90:16-130 for one tile (the wpxyz halo exchange is a no-op on one tile
 // of a LAM grid: the 2-point halo is never read, see DESIGN notes).
+// klo..khi: only the position sets of these levels are produced, from the departure
+// points of levels klo-1..khi+2 (each point's iteration reads only the winds and its own
+// iterate, so a band of levels is exact); 1..nk is the whole tile.
 template <class S>
 inline void traject(const Grid& g, double dt, int schm_advec, double bA_m, double bA, int niter, int order,
                     const float* uu_ext, const float* vv_ext, const float* ww_ext, const float* ut0,
-                    const float* vt0, const float* zdt0, TrajState<S>* s) {
+                    const float* vt0, const float* zdt0, TrajState<S>* s, int klo = 1, int khi = 1 << 30) {
   Dt t = dt_split(dt, schm_advec, bA_m, bA);
   prepare_winds<S>(g, uu_ext, vv_ext, ww_ext, ut0, vt0, zdt0, s);
   int k00 = g.k0m;
   if (g.k0 > 1) k00 = 1;
+  const int ka = std::max(k00, klo - 1), kb = std::min(g.nk, khi + 2);
   // each arrival point only reads the static winds and its own iterate, so the points of
   // one iteration are independent (the reference shares them out per level, :67-68)
   for (int it = 1; it <= niter; it++) {
 #pragma omp parallel for collapse(2) schedule(static)
-    for (int k = k00; k <= g.nk; k++)
+    for (int k = ka; k <= kb; k++)
       for (int j = 1; j <= g.nj; j++)
         for (int i = 1; i <= g.ni; i++) traj_point<S>(g, t, i, j, k, order, s);
   }
-  for (int k = g.k0; k <= g.nk; k++)
+  for (int k = std::max(g.k0, klo); k <= std::min(g.nk, khi); k++)
     for (int j = g.j0; j <= g.jn; j++)
       for (int i = g.i0; i <= g.in; i++) {
         long q = box_idx(g.i0, g.in, g.j0, g.jn, g.k0, i, j, k), r = p3_idx(g.ni, g.nj, g.k0m, i, j, k);
         for (int c = 0; c < 3; c++) s->pm[q + c] = s->pxyzm[r + c];
       }
-  interp_traj<S>(g, t, dt, zdt0, s);
+  interp_traj<S>(g, t, dt, zdt0, s, klo, khi);
 }
 
 // ------------------------------ Bermejo-Conde -------------------------------------

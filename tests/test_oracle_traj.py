@@ -397,3 +397,22 @@ def test_departure_zetadot_closed_form():
     # planted: the departure term with dtzA instead of dtzD
     wrong = (zm[1:nk + 1] + dtzA * c1 * zx[0]) / (1.0 + dtzA * c1)
     assert np.abs(pm[ok][:, 0, 0, 2] - _frac_index(wrong[ok], zm, nk)).max() > 1e-3
+
+
+def test_level_band_equals_full_run():
+    """The oracle's level band (used at the largest tile to keep the CPU time bounded):
+    each point's iteration reads only the winds and its own iterate, and the
+    thermodynamic sets read levels k-1..k+2, so the position sets of a band equal those of
+    the whole-tile run bit for bit, at the top, inside and at the bottom."""
+    cfg = small_cfg(jets=True, cmax_xy=3.0, cmax_z=2.0, seed=9)
+    g = S.grid_desc(cfg)
+    G = O.Grid(g)
+    w = {k: v.numpy() for k, v in S.winds(cfg, g).items()}
+    full = G.traject(w, cfg.dt, storage="f32")
+    i, nk = G.info, cfg.nk
+    for klo, khi in ((1, 4), (7, 12), (nk - 3, nk)):
+        b = G.traject(w, cfg.dt, storage="f32", band=(klo, khi), want=("pm", "pmu", "pmv", "pt", "pb"))
+        for name, k0 in (("pm", i["k0"]), ("pmu", i["k0"]), ("pmv", i["k0"]), ("pt", i["k0t"]), ("pb", 1)):
+            per = full[name].size // (nk - k0 + 1)
+            sl = slice((klo - k0) * per, (khi - k0 + 1) * per)
+            assert np.array_equal(full[name][sl], b[name][sl]), (name, klo, khi)
