@@ -91,16 +91,14 @@ struct ItemFields {  // field pointer tables of a plan item (device arrays in gl
 #undef ADZ_CONST
 };
 
-// Evaluate one target from its stencil accessor and store the outputs at dst offset o.
-template <class TF, int MATH, int POST, class Fields, class Acc>
-__device__ __forceinline__ void eval_store_a(const Fields& F, int f, Acc at, const void* cw, long o) {
+// Narrow and store one target's outputs at dst offset o: the value (clipped to [min, max]
+// with ADZ_POST_CLIP) and, with ADZ_POST_STORE, lin clipped to [min, max], min and max.
+// TC is the arithmetic type (FP64 for the reference's arithmetic, FP32 for the mixed form).
+template <class TF, int MATH, int POST, class Fields, class TC>
+__device__ __forceinline__ void store_out(const Fields& F, int f, long o, TC v, TC l, TC mi, TC ma) {
   typename GPtr<TF>::type d = (typename GPtr<TF>::type)F.d(f);
   if (MATH == ADZ_MATH_REF) {
-    const double* c = (const double*)cw;
-    double v = tricub_ref_a<TF>(at, c);
     if (POST) {
-      double l, mi, ma;
-      mono_extra_a<TF, double>(at, c, &l, &mi, &ma);
       // adz_interp_hlt.F90:158-160 narrows each to real before clipping lin
       TF ls = from_f64<TF>(l), mis = from_f64<TF>(mi), mas = from_f64<TF>(ma);
       TF vs = from_f64<TF>(v);
@@ -115,11 +113,7 @@ __device__ __forceinline__ void eval_store_a(const Fields& F, int f, Acc at, con
       d[o] = from_f64<TF>(v);
     }
   } else {
-    const float* c = (const float*)cw;
-    float v = tricub_f32_a<TF>(at, c);
     if (POST) {
-      float l, mi, ma;
-      mono_extra_a<TF, float>(at, c, &l, &mi, &ma);
       TF ls = from_f32<TF>(l), mis = from_f32<TF>(mi), mas = from_f32<TF>(ma);
       TF vs = from_f32<TF>(v);
       if (POST & ADZ_POST_CLIP) vs = from_f32<TF>(fmaxf(to_f32(mis), fminf(to_f32(mas), to_f32(vs))));
@@ -132,6 +126,22 @@ __device__ __forceinline__ void eval_store_a(const Fields& F, int f, Acc at, con
     } else {
       d[o] = from_f32<TF>(v);
     }
+  }
+}
+
+// Evaluate one target from its stencil accessor and store the outputs at dst offset o.
+template <class TF, int MATH, int POST, class Fields, class Acc>
+__device__ __forceinline__ void eval_store_a(const Fields& F, int f, Acc at, const void* cw, long o) {
+  if (MATH == ADZ_MATH_REF) {
+    const double* c = (const double*)cw;
+    double v = tricub_ref_a<TF>(at, c), l = 0, mi = 0, ma = 0;
+    if (POST) mono_extra_a<TF, double>(at, c, &l, &mi, &ma);
+    store_out<TF, MATH, POST>(F, f, o, v, l, mi, ma);
+  } else {
+    const float* c = (const float*)cw;
+    float v = tricub_f32_a<TF>(at, c), l = 0, mi = 0, ma = 0;
+    if (POST) mono_extra_a<TF, float>(at, c, &l, &mi, &ma);
+    store_out<TF, MATH, POST>(F, f, o, v, l, mi, ma);
   }
 }
 
@@ -241,8 +251,8 @@ __device__ __forceinline__ void per_lane(const DevItem& it, const Fields& F, int
 //     working set its L2 can hold.  Tiles past the last row (njb not a multiple of S) hold
 //     no valid target.
 __device__ __forceinline__ void tile_coords(const DevItem& it, long ptile, int level_inner, int* ib, int* jb,
-                                            int* kk) {
-  const long nib = (it.nx + kTI - 1) / kTI, njb = (it.ny + kTJ - 1) / kTJ;
+                                            int* kk, int tw = kTI) {
+  const long nib = (it.nx + tw - 1) / tw, njb = (it.ny + kTJ - 1) / kTJ;
   if (level_inner >= 2) {
     const long S = level_inner, per = (long)it.nz * S;
     const long sc = ptile / per, r = ptile % per;
@@ -261,8 +271,8 @@ __device__ __forceinline__ void tile_coords(const DevItem& it, long ptile, int l
 }
 
 // tiles of an item in the order above
-inline long item_ptiles(int nx, int ny, int nz, int level_inner) {
-  const long nib = (nx + kTI - 1) / kTI, njb = (ny + kTJ - 1) / kTJ;
+inline long item_ptiles(int nx, int ny, int nz, int level_inner, int tw = kTI) {
+  const long nib = (nx + tw - 1) / tw, njb = (ny + kTJ - 1) / kTJ;
   if (level_inner >= 2) return nib * ((njb + level_inner - 1) / level_inner) * level_inner * nz;
   return nib * njb * nz;
 }
@@ -614,6 +624,212 @@ __global__ __launch_bounds__(kPCThreads) void k_plan_pc(PlanArgs A) {
   }
 }
 
+// ---------------------------------------------------------------------------------
+// Staging mode 4 (FP32 sources): stencil columns shared by the lanes of a wavefront.
+//
+// The targets of a wave are consecutive along i.  Where target l+c sits c source cells to
+// the right of target l in the same row and vertical interval, column c of target l's
+// stencil is column 0 of target l+c's.  Each lane therefore loads only column 0 of its own
+// stencil (16 dword loads: 4 rows x 4 planes, coalesced across the wave) and reduces that
+// column four times, over z and then y, with the weights of targets l, l-1, l-2 and l-3:
+// the per-column part of the reference's operation order, so each partial is bit for bit
+// what the owning target computes for that column itself.  The partials travel to their
+// targets with three lane shifts; a column's min/max (target independent) with two.  The
+// x combination then runs at the target in the usual order.  A tile is kSW = 61 targets
+// wide, so lanes 61..63 load the columns 1..3 of the wave's last target.  A target whose
+// right-hand neighbours do not continue its stencil (a cell skipped or repeated, another
+// row or interval, a stencil out of range) is evaluated by per-lane row loads, as
+// staging mode 2 does, in the same arithmetic.
+constexpr int kSW = 61;
+
+template <class T> __device__ __forceinline__ T fmx(T a, T b, T c);
+template <> __device__ __forceinline__ float fmx<float>(float a, float b, float c) { return fmaf(a, b, c); }
+template <> __device__ __forceinline__ double fmx<double>(double a, double b, double c) { return fma(a, b, c); }
+
+// Weight slots one column reduction needs: y (4..7), z (8..11), linear y (16, 17), linear
+// z (18, 19).
+__device__ __forceinline__ constexpr int col_slot(int q) { return q < 8 ? 4 + q : 8 + q; }
+
+// z then y over one column v[r][z] (rows r, planes z) with weights w[col_slot(.)]
+template <class TC>
+__device__ __forceinline__ TC col_tricub(const TC (&v)[4][4], const TC (&w)[12]) {
+  TC ya = 0;
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+    TC za = v[r][0] * w[4];
+    za = fmx<TC>(v[r][1], w[5], za);
+    za = fmx<TC>(v[r][2], w[6], za);
+    za = fmx<TC>(v[r][3], w[7], za);
+    ya = (r == 0) ? za * w[0] : fmx<TC>(za, w[r], ya);
+  }
+  return ya;
+}
+
+// the linear part of one column: rows 1..2, planes 1..2
+template <class TC>
+__device__ __forceinline__ TC col_lin(const TC (&v)[4][4], const TC (&w)[12]) {
+  TC vz1 = v[1][1] * w[10];
+  vz1 = fmx<TC>(v[1][2], w[11], vz1);
+  TC vy = vz1 * w[8];
+  TC vz2 = v[2][1] * w[10];
+  vz2 = fmx<TC>(v[2][2], w[11], vz2);
+  return fmx<TC>(vz2, w[9], vy);
+}
+
+// min / max of one column over rows 0..3 and planes 1..2: the operand rule of MINPD /
+// MAXPD in FP64 (as mono_extra_a), IEEE minNum / maxNum in FP32
+template <class TC>
+__device__ __forceinline__ void col_minmax(const TC (&v)[4][4], TC* mi, TC* ma) {
+  if constexpr (std::is_same<TC, float>::value) {
+    float lo = fminf(v[0][1], v[0][2]), hi = fmaxf(v[0][1], v[0][2]);
+#pragma unroll
+    for (int r = 1; r < 4; r++) {
+      lo = fminf(lo, fminf(v[r][1], v[r][2]));
+      hi = fmaxf(hi, fmaxf(v[r][1], v[r][2]));
+    }
+    *mi = lo;
+    *ma = hi;
+  } else {
+    TC lo = v[0][1], hi = v[0][1];
+    lo = (lo < v[0][2]) ? lo : v[0][2];
+    hi = (hi > v[0][2]) ? hi : v[0][2];
+#pragma unroll
+    for (int r = 1; r < 4; r++) {
+      lo = (lo < v[r][1]) ? lo : v[r][1];
+      hi = (hi > v[r][1]) ? hi : v[r][1];
+      lo = (lo < v[r][2]) ? lo : v[r][2];
+      hi = (hi > v[r][2]) ? hi : v[r][2];
+    }
+    *mi = lo;
+    *ma = hi;
+  }
+}
+
+template <int MATH, int POST, class TP, class Fields>
+__device__ __forceinline__ void shared_tile(const DevItem& it, const Fields& F, long ptile, int f0, int f1,
+                                            int level_inner) {
+  typedef typename std::conditional<MATH == ADZ_MATH_REF, double, float>::type TC;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  int ib, jb, kk;
+  tile_coords(it, ptile, level_inner, &ib, &jb, &kk, kSW);
+  const int jj = jb * kTJ + wv;
+  if (jj >= it.ny) return;  // a row past the item: this wave has no target (no barrier follows)
+  const int ii = ib * kSW + lane;
+  const bool target = lane < kSW && ii < it.nx;
+  const int i = it.i0 + ii, j = it.j0 + jj, k = it.k0 + kk;
+  const long p = (long)ii + (long)it.nx * (jj + (long)it.ny * kk);
+  const long o = (long)(i - 1) + (long)it.dni * ((j - 1) + (long)it.dnj * (k - 1));
+  Index x;
+  bool ok = false;
+  if (target) {
+    double px, py, pz;
+    load_pos<TP>(it.xyz, p, &px, &py, &pz);
+    ok = index_step(px, py, pz, it.lv, &x);
+  }
+  double cw[20];  // own weights (FP64, or FP32 in the storage of the first 10 slots)
+  if (ok) {
+    if (MATH == ADZ_MATH_REF) weights_ref(x, it.lv, cw);
+    else weights_f32(x, it.lv, (float*)cw);
+  }
+  const TC* c = (const TC*)cw;
+  if (!it.span) {  // sources too large for 32-bit buffer offsets: per-lane global loads
+    per_lane<float, MATH, POST>(it, F, f0, f1, x, ok, target, cw, o);
+    return;
+  }
+  // stencil corner of each lane (element offset from minoff), its column, interval kind
+  int e = ok ? (int)(x.off - it.minoff) : 0, ix = ok ? x.ix : 0, zl = ok ? x.zlinear : 0, ld = ok ? 1 : 0;
+  // lanes past the wave's last target continue its stencil to the right (its columns 1..3)
+  const int last = min(kSW - 1, it.nx - 1 - ib * kSW);
+  {
+    const int eL = __shfl(e, last, 64), ixL = __shfl(ix, last, 64), zlL = __shfl(zl, last, 64),
+              okL = __shfl(ld, last, 64);
+    if (lane > last) {
+      const int d = lane - last;
+      e = eL + d; ix = ixL + d; zl = zlL; ld = (okL && d <= 3) ? 1 : 0;
+    }
+  }
+  // does column c of this target come from lane + c?
+  bool fast = ok;
+#pragma unroll
+  for (int cc = 1; cc < 4; cc++) {
+    const int en = __shfl_down(e, cc, 64), ixn = __shfl_down(ix, cc, 64), zln = __shfl_down(zl, cc, 64),
+              ldn = __shfl_down(ld, cc, 64);
+    fast = fast && ldn && en == e + cc && ixn == ix + cc && zln == zl;
+  }
+  const bool slow_any = __ballot(target && ok && !fast) != 0;
+  // weights of targets lane-1, lane-2, lane-3 for their columns 1, 2, 3 held here
+  TC w[4][12];
+#pragma unroll
+  for (int q = 0; q < 12; q++) w[0][q] = c[col_slot(q)];
+#pragma unroll
+  for (int cc = 1; cc < 4; cc++)
+#pragma unroll
+    for (int q = 0; q < 12; q++) w[cc][q] = __shfl_up(w[0][q], cc, 64);
+  // row offsets of the 4 planes of row 0; rows r add r*ni (uniform: the scalar offset)
+  int bo[4];
+  {
+    const int nij = (int)it.lv.nij;
+    const int o1 = zl ? 0 : nij, o2 = o1 + nij, o3 = o2 + o1;
+    bo[0] = e * 4; bo[1] = (e + o1) * 4; bo[2] = (e + o2) * 4; bo[3] = (e + o3) * 4;
+  }
+  const int rstride = (int)(it.lv.ni * 4);
+  for (int f = f0; f < f1; f++) {
+    TC Y[4] = {0, 0, 0, 0}, Lc[4] = {0, 0, 0, 0}, cmn = 0, cmx = 0;
+    if (ld) {
+      const __amdgpu_buffer_rsrc_t rs = field_rsrc((const float*)F.s(f) + it.minoff, it.span);
+      TC v[4][4];
+#pragma unroll
+      for (int r = 0; r < 4; r++)
+#pragma unroll
+        for (int z = 0; z < 4; z++)
+          v[r][z] = (TC)__builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, bo[z], r * rstride, 0));
+#pragma unroll
+      for (int cc = 0; cc < 4; cc++) Y[cc] = col_tricub<TC>(v, w[cc]);
+      if (POST) {
+#pragma unroll
+        for (int cc = 0; cc < 4; cc++) Lc[cc] = col_lin<TC>(v, w[cc]);
+        col_minmax<TC>(v, &cmn, &cmx);
+      }
+    }
+    // column cc of target `lane` was reduced by lane + cc
+#pragma unroll
+    for (int cc = 1; cc < 4; cc++) Y[cc] = __shfl_down(Y[cc], cc, 64);
+    TC mi1 = 0, mi2 = 0, ma1 = 0, ma2 = 0;
+    if (POST) {
+#pragma unroll
+      for (int cc = 1; cc < 4; cc++) Lc[cc] = __shfl_down(Lc[cc], cc, 64);
+      mi1 = __shfl_down(cmn, 1, 64); mi2 = __shfl_down(cmn, 2, 64);
+      ma1 = __shfl_down(cmx, 1, 64); ma2 = __shfl_down(cmx, 2, 64);
+    }
+    if (fast) {
+      TC v;
+      if (MATH == ADZ_MATH_REF) {
+        const TC y0 = Y[0] * c[0], y1 = Y[1] * c[1], y2 = Y[2] * c[2], y3 = Y[3] * c[3];
+        v = (y0 + y2) + (y1 + y3);
+      } else {
+        v = fmx<TC>(Y[3], c[3], fmx<TC>(Y[2], c[2], fmx<TC>(Y[1], c[1], Y[0] * c[0])));
+      }
+      TC l = 0, mi = 0, ma = 0;
+      if (POST) {
+        const TC l0 = Lc[0] * c[12], l1 = Lc[1] * c[13], l2 = Lc[2] * c[14], l3 = Lc[3] * c[15];
+        l = (l0 + l2) + (l1 + l3);
+        if (MATH == ADZ_MATH_REF) {
+          mi = (mi1 < mi2) ? mi1 : mi2;
+          ma = (ma1 > ma2) ? ma1 : ma2;
+        } else {
+          mi = fminf(mi1, mi2);
+          ma = fmaxf(ma1, ma2);
+        }
+      }
+      store_out<float, MATH, POST>(F, f, o, v, l, mi, ma);
+    } else if (target && !ok) {
+      store_nan<float, MATH, POST>(F, f, o);
+    }
+  }
+  // targets whose stencil is not continued by their neighbours: per-lane row loads
+  if (slow_any) per_lane<float, MATH, POST>(it, F, f0, f1, x, target && ok && !fast, false, cw, o);
+}
+
 // Persistent loop over the tile list of a plan.
 template <class TF, class TP, int MATH, int POST, int STAGE>
 __device__ __forceinline__ void plan_loop(const PlanArgs& A, TF* boxA, TF* boxB, int* red, int* rowoff) {
@@ -637,13 +853,16 @@ __device__ __forceinline__ void plan_loop(const PlanArgs& A, TF* boxA, TF* boxB,
     else { grp = local / it.ptiles; ptile = local % it.ptiles; }
     int f0 = (int)grp * A.g, f1 = min(it.nf, f0 + A.g);
     ItemFields F{it.src, it.dst, it.lin, it.mn, it.mx};
-    run_tile<TF, TP, MATH, POST, ItemFields>(it, F, ptile, f0, f1, boxA, boxB, red, rowoff, STAGE, A.level_inner);
+    if constexpr (STAGE == 4 && std::is_same<TF, float>::value)
+      shared_tile<MATH, POST, TP>(it, F, ptile, f0, f1, A.level_inner);
+    else
+      run_tile<TF, TP, MATH, POST, ItemFields>(it, F, ptile, f0, f1, boxA, boxB, red, rowoff, STAGE, A.level_inner);
   }
 }
 
-// Persistent kernel: STAGE 1 stages each tile's source box in LDS, STAGE 2 reads per lane
-// (separate instances, so the per-lane one carries neither the LDS buffers nor the
-// staging path's registers).
+// Persistent kernel: STAGE 1 stages each tile's source box in LDS, STAGE 2 reads per lane,
+// STAGE 4 shares stencil columns across the lanes of a wave (separate instances, so the
+// per-lane ones carry neither the LDS buffers nor the staging path's registers).
 template <class TF, class TP, int MATH, int POST, int STAGE>
 __global__ __launch_bounds__(kTile) void k_plan(PlanArgs A) {
   if constexpr (STAGE == 1) {
@@ -653,7 +872,7 @@ __global__ __launch_bounds__(kTile) void k_plan(PlanArgs A) {
     __shared__ int rowoff[kMaxRows];
     plan_loop<TF, TP, MATH, POST, 1>(A, boxA, boxB, red, rowoff);
   } else {
-    plan_loop<TF, TP, MATH, POST, 2>(A, nullptr, nullptr, nullptr, nullptr);
+    plan_loop<TF, TP, MATH, POST, STAGE>(A, nullptr, nullptr, nullptr, nullptr);
   }
 }
 
@@ -711,6 +930,14 @@ bool fill_item(DevItem* d, const adz_ztab* geom, const void* xyz, int i0, int in
 template <class TF, class TP, int M>
 void* pick_plan(int post, int stage) {
   if constexpr (std::is_same<TF, float>::value) {
+    if (stage == 4) {
+      switch (post) {
+        case 0: return (void*)&k_plan<TF, TP, M, 0, 4>;
+        case 1: return (void*)&k_plan<TF, TP, M, 1, 4>;
+        case 2: return (void*)&k_plan<TF, TP, M, 2, 4>;
+        default: return (void*)&k_plan<TF, TP, M, 3, 4>;
+      }
+    }
     if (stage == 3) {
       switch (post) {
         case 0: return (void*)&k_plan_pc<TP, M, 0>;
@@ -762,6 +989,9 @@ void* pick(int fdt, int pdt, int math, int post, int stage = 2) {
 #undef P_
   return nullptr;
 }
+
+// targets per tile along i: kSW for the shared-column staging of FP32 sources, else kTI
+int tile_width(int stage, int fdtype) { return (stage == 4 && fdtype == ADZ_F32) ? kSW : kTI; }
 
 int device_cus() {
   static int cus = 0;
@@ -847,7 +1077,7 @@ int adz_plan_create(const adz_item* items, int nitems, int fdtype, int math, int
       ptrs[4 * nptr_total + pofs + f] = a.fld[f].mx;
     }
     dv[q].ngroups = (a.nptr + T.fields_per_pass - 1) / T.fields_per_pass;
-    dv[q].ptiles = item_ptiles(dv[q].nx, dv[q].ny, dv[q].nz, T.level_inner);
+    dv[q].ptiles = item_ptiles(dv[q].nx, dv[q].ny, dv[q].nz, T.level_inner, tile_width(T.stage, fdtype));
     dv[q].tile0 = tiles;
     tiles += dv[q].ptiles * dv[q].ngroups;
     dv[q].src = (const void* const*)(intptr_t)(0 * nptr_total + pofs);  // patched below
@@ -954,7 +1184,7 @@ int adz_plan_autotune(const adz_item* items, int nitems, int fdtype, int math, i
   cur.fields_per_pass = std::min(cur.fields_per_pass, maxf);
   float cur_ms = 0.f;
   int rc = time_one(cur, &cur_ms);
-  const int stages[] = {1, 2, 3}, fpps[] = {2, 4, 8, 16, 32, 64}, bpcs[] = {2, 4, 8, 16}, flags[] = {0, 1},
+  const int stages[] = {1, 2, 3, 4}, fpps[] = {2, 4, 8, 16, 32, 64}, bpcs[] = {2, 4, 8, 16}, flags[] = {0, 1},
             orders[] = {0, 1, 2, 4, 8};
   // coordinates: 0 staging, 1 tile order, 2 fields per pass, 3 blocks per CU, 4 loop order,
   // 5 XCD chunking; the tile order is searched again after fields per pass (they interact
@@ -963,7 +1193,7 @@ int adz_plan_autotune(const adz_item* items, int nitems, int fdtype, int math, i
   for (int ci = 0; ci < 7 && rc == ADZ_OK; ci++) {
     const int coord = coords[ci];
     const int* vals = coord == 0 ? stages : coord == 1 ? orders : coord == 2 ? fpps : coord == 3 ? bpcs : flags;
-    int nv = coord == 0 ? (fdtype == ADZ_F32 ? 3 : 2) : coord == 1 ? 5 : coord == 2 ? 6 : coord == 3 ? 4 : 2;
+    int nv = coord == 0 ? (fdtype == ADZ_F32 ? 4 : 2) : coord == 1 ? 5 : coord == 2 ? 6 : coord == 3 ? 4 : 2;
     for (int v = 0; v < nv && rc == ADZ_OK; v++) {
       adz_tune t = cur;
       int* slot = coord == 0 ? &t.stage : coord == 1 ? &t.level_inner : coord == 2 ? &t.fields_per_pass

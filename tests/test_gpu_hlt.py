@@ -243,3 +243,116 @@ def test_stage3_waits_that_give_up_fall_back_exactly(T):
     plan.free()
     for c in keep:
         c.free()
+
+
+@pytest.mark.parametrize("post", [0, 1, 3])
+@pytest.mark.parametrize("math", ["f32", "ref"])
+@pytest.mark.parametrize("pdt", ["f32", "f64"])
+def test_shared_columns_match_single_calls(T, post, math, pdt):
+    """Staging mode 4 (stencil columns shared across a wave, 61-target tiles) against one
+    adz_tricub_hlt call per item (per-lane stencils): bit for bit, with lin/min/max stored
+    and clipped, in both arithmetics, with FP32 and FP64 positions, over items whose winds
+    reach Courant 3 (targets whose neighbours skip or repeat a cell take the per-lane
+    path) and over every tile order."""
+    torch, L, A = T
+    M = L.ADZ_MATH_F32 if math == "f32" else L.ADZ_MATH_REF
+    P = L.ADZ_F32 if pdt == "f32" else L.ADZ_F64
+    cfgs = [S.Config("s0", 130, 37, 12, 4, 300.0, jets=True, cmax_xy=3.0, cmax_z=1.0, seed=81),
+            S.Config("s1", 61, 20, 9, 4, 300.0, seed=82),
+            S.Config("s2", 200, 9, 10, 4, 300.0, jets=True, cmax_xy=1.0, seed=83)]
+    items, refs, keep = [], [], []
+    for cfg in cfgs:
+        g, ctx, src64 = _setup(T, cfg, P, "f32", nfields=3, seed=cfg.seed)
+        keep.append(ctx)
+        i = ctx.info
+        box = (i["i0"], i["in"], i["j0"], i["jn"], i["k0t"], cfg.nk)
+        n_model = (cfg.ni + 2 * MH) * (cfg.nj + 2 * MH) * cfg.nk
+        srcs = [torch.from_numpy(s).float().cuda() for s in src64]
+
+        def stack_of():
+            d = {"dst": torch.full((n_model,), -7.0, device="cuda")}
+            if post & 1:
+                d.update({k: torch.full((n_model,), -7.0, device="cuda") for k in ("lin", "min", "max")})
+            return d
+        stack = [dict(src=s, **stack_of()) for s in srcs]
+        ref = [dict(src=s, **stack_of()) for s in srcs]
+        src_off = (1 - i["lminx"]) + i["nit"] * (1 - i["lminy"])
+        dl = (1 - MH, cfg.ni + MH, 1 - MH, cfg.nj + MH)
+        dst_off = (1 - dl[0]) + (dl[1] - dl[0] + 1) * (1 - dl[2])
+        A.adz_tricub_hlt(ref, ctx.array(A.ADZ_A_PT), ctx.ztab(True), *box[:5], box[5], dl, math=M, post=post,
+                         src_off=src_off)
+        items.append(dict(geom=ctx.ztab(True), xyz=ctx.array(A.ADZ_A_PT), box=box, dlayout=dl,
+                          fields=A.field_list(stack, src_off, dst_off)))
+        refs.append((stack, ref))
+    for tune in (dict(fields_per_pass=2, points_outer=0, blocks_per_cu=4, xcd_chunks=1, stage=4),
+                 dict(fields_per_pass=3, points_outer=1, blocks_per_cu=2, xcd_chunks=0, stage=4, level_inner=1),
+                 dict(fields_per_pass=1, points_outer=0, blocks_per_cu=3, xcd_chunks=1, stage=4, level_inner=3)):
+        for stack, _ in refs:
+            for a in stack:
+                for k in ("dst", "lin", "min", "max"):
+                    if k in a:
+                        a[k].fill_(-7.0)
+        plan = A.Plan(items, L.ADZ_F32, math=M, post=post, tune=tune)
+        assert plan.tune()["stage"] == 4
+        plan.execute()
+        torch.cuda.synchronize()
+        for stack, ref in refs:
+            for a, b in zip(stack, ref):
+                for k in ("dst", "lin", "min", "max"):
+                    if k in a:
+                        assert torch.equal(a[k], b[k]), (tune, k)
+        plan.free()
+    for c in keep:
+        c.free()
+
+
+@pytest.mark.parametrize("math", ["f32", "ref"])
+def test_shared_columns_edge_positions(T, math):
+    """Staging mode 4 with positions that break the lane sharing on purpose: stencils out
+    of range (NaN targets whose lanes serve no neighbour), positions on cell faces, a cell
+    repeated or skipped every few targets, levels clamped at both ends.  Bit for bit the
+    single-call result (NaN patterns included), with lin/min/max."""
+    torch, L, A = T
+    M = L.ADZ_MATH_F32 if math == "f32" else L.ADZ_MATH_REF
+    cfg = S.Config("e", 150, 12, 10, 4, 300.0, seed=85)
+    g, ctx, src64 = _setup(T, cfg, L.ADZ_F32, "f32", nfields=2, seed=cfg.seed)
+    i = ctx.info
+    box = (i["i0"], i["in"], i["j0"], i["jn"], i["k0t"], cfg.nk)
+    nx, ny, nz = box[1] - box[0] + 1, box[3] - box[2] + 1, box[5] - box[4] + 1
+    rng = np.random.default_rng(9)
+    ii = np.arange(box[0], box[1] + 1, dtype=np.float64)[None, None, :]
+    jj = np.arange(box[2], box[3] + 1, dtype=np.float64)[None, :, None]
+    kk = np.arange(box[4], box[5] + 1, dtype=np.float64)[:, None, None]
+    px = ii - 0.3 + 0.45 * np.sin(ii * 0.7) + 0.0 * jj + 0.0 * kk          # repeats and skips along i
+    py = jj + 0.25 + 0.0 * ii + 0.0 * kk
+    pz = kk + 0.4 + 0.0 * ii + 0.0 * jj
+    px = np.where((np.arange(nx) % 17 == 5)[None, None, :], np.floor(px), px)   # on cell faces
+    pz = np.where((np.arange(nx) % 23 == 3)[None, None, :], -3.0, pz)           # below the first level
+    pz = np.where((np.arange(nx) % 29 == 4)[None, None, :], cfg.nk + 5.0, pz)   # above the last
+    bad = rng.random((nz, ny, nx)) < 0.03
+    px = np.where(bad, 1.0e10, px)                                              # stencil out of range
+    px = np.where((np.arange(nx) % 41 == 40)[None, None, :], 1 - 40.0, px)      # left of the source
+    pos = np.stack([px, py, pz], -1).astype(np.float32).reshape(-1)
+    xyz = torch.from_numpy(pos).cuda()
+    n_model = (cfg.ni + 2 * MH) * (cfg.nj + 2 * MH) * cfg.nk
+    srcs = [torch.from_numpy(s).float().cuda() for s in src64]
+    mk = lambda: {k: torch.full((n_model,), -7.0, device="cuda") for k in ("dst", "lin", "min", "max")}
+    stack = [dict(src=s, **mk()) for s in srcs]
+    ref = [dict(src=s, **mk()) for s in srcs]
+    src_off = (1 - i["lminx"]) + i["nit"] * (1 - i["lminy"])
+    dl = (1 - MH, cfg.ni + MH, 1 - MH, cfg.nj + MH)
+    dst_off = (1 - dl[0]) + (dl[1] - dl[0] + 1) * (1 - dl[2])
+    A.adz_tricub_hlt(ref, xyz, ctx.ztab(True), *box[:5], box[5], dl, math=M, post=3, src_off=src_off)
+    item = dict(geom=ctx.ztab(True), xyz=xyz, box=box, dlayout=dl, fields=A.field_list(stack, src_off, dst_off))
+    plan = A.Plan([item], L.ADZ_F32, math=M, post=3,
+                  tune=dict(fields_per_pass=2, points_outer=0, blocks_per_cu=2, xcd_chunks=0, stage=4))
+    plan.execute()
+    torch.cuda.synchronize()
+    nan_seen = False
+    for a, b in zip(stack, ref):
+        for k in ("dst", "lin", "min", "max"):
+            assert torch.equal(a[k].view(torch.int32), b[k].view(torch.int32)), k
+            nan_seen = nan_seen or bool(torch.isnan(a[k]).any())
+    assert nan_seen
+    plan.free()
+    ctx.free()
