@@ -439,6 +439,11 @@ int adz_set(const adz_grid* g, int pos_dtype, int itraj, adz_ctx** out) {
     delete c;
     return fail(ADZ_EINVAL, "adz_set: pilot widths too small for the interpolation boxes (need pil >= 6 on edges)");
   }
+  if (P.in < P.i0 || P.jn < P.j0 || P.inu < P.i0u || P.jnv < P.j0v || P.inb < P.i0b || P.jnb < P.j0b ||
+      P.k0 > nk || P.k0t > nk) {
+    delete c;
+    return fail(ADZ_EINVAL, "adz_set: tile too small for its pilot zones (an interpolation box is empty)");
+  }
   c->zm.assign(g->zm, g->zm + nk + 2); c->zt.assign(g->zt, g->zt + nk + 2); c->zx.assign(g->zx, g->zx + nk + 1);
   const auto& zm = c->zm; const auto& zt = c->zt; const auto& zx = c->zx;
   P.sig = (zm[nk] - zm[1]) / std::fabs(zm[nk] - zm[1]);

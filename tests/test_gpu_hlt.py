@@ -258,8 +258,8 @@ def test_shared_columns_match_single_calls(T, post, math, pdt):
     M = L.ADZ_MATH_F32 if math == "f32" else L.ADZ_MATH_REF
     P = L.ADZ_F32 if pdt == "f32" else L.ADZ_F64
     cfgs = [S.Config("s0", 130, 37, 12, 4, 300.0, jets=True, cmax_xy=3.0, cmax_z=1.0, seed=81),
-            S.Config("s1", 61, 20, 9, 4, 300.0, seed=82),
-            S.Config("s2", 200, 9, 10, 4, 300.0, jets=True, cmax_xy=1.0, seed=83)]
+            S.Config("s1", 61, 26, 9, 4, 300.0, seed=82),
+            S.Config("s2", 200, 24, 10, 4, 300.0, jets=True, cmax_xy=1.0, seed=83)]
     items, refs, keep = [], [], []
     for cfg in cfgs:
         g, ctx, src64 = _setup(T, cfg, P, "f32", nfields=3, seed=cfg.seed)
@@ -314,7 +314,7 @@ def test_shared_columns_edge_positions(T, math):
     single-call result (NaN patterns included), with lin/min/max."""
     torch, L, A = T
     M = L.ADZ_MATH_F32 if math == "f32" else L.ADZ_MATH_REF
-    cfg = S.Config("e", 150, 12, 10, 4, 300.0, seed=85)
+    cfg = S.Config("e", 150, 26, 10, 4, 300.0, seed=85)
     g, ctx, src64 = _setup(T, cfg, L.ADZ_F32, "f32", nfields=2, seed=cfg.seed)
     i = ctx.info
     box = (i["i0"], i["in"], i["j0"], i["jn"], i["k0t"], cfg.nk)
