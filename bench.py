@@ -60,6 +60,9 @@ def parse():
                         "arithmetic and FP64 position history), f64 = FP64 throughout (timed beside it either way)")
     p.add_argument("--tiles", default=os.path.join(ROOT, "gem_amd", "tuned_tiles.json"),
                    help="JSON file of tile parameters per workload (the autotuner's committed result)")
+    p.add_argument("--fortran", action="store_true",
+                   help="c5 through the Fortran drop-in layer instead (module adz_gpu: adz_main + adz_tracers "
+                        "before and after the pressure adjustment, one hipGraph), timed alone")
     p.add_argument("--retune", action="store_true",
                    help="run adz_plan_autotune instead of using the tiles it found before (gem_amd/tuned_tiles.json)")
     return p.parse_args()
@@ -83,6 +86,8 @@ def main():
     dev = torch.device("cuda:0")
     if args.config == "c4":
         return run_c4(args, np, torch, L, A, S, dev)
+    if args.fortran:
+        return run_fortran(args, np, torch, L, A, S, dev)
     cfg = S.CONFIGS[args.config]
     g = S.grid_desc(cfg)
     nf = cfg.nfields
@@ -247,8 +252,8 @@ def main():
         y = torch.empty_like(x)
         y.copy_(x)
         torch.cuda.synchronize()
-        print(json.dumps({"profile_kernel": "k_plan (tracer group)", "tiles": tuneB, "copy_bytes": 2 * 4 * 2 ** 28}),
-              flush=True)
+        print(json.dumps({"profile_kernel": "k_plan (tracer group)", "tiles": tuneB, "copy_bytes": 2 * 4 * 2 ** 28,
+                          "kernel_source_sha16": kernel_source_sha()}), flush=True)
         return
 
     # ------------------------------------------------------------------ timed step (hipGraph)
@@ -431,9 +436,11 @@ def main():
     torch.cuda.empty_cache()
 
     # ------------------------------------------------------------------ CPU baseline
-    cpu = bc_cpu = None
+    cpu = bc_cpu = traj_cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(np, torch, cfg, g, src, pt, tbox, mod_shape, nf, args.cpu_seconds)
+        traj_cpu = traject_baseline(np, torch, L, A, S, dev)
+        traj_cpu["gpu_mpoints_per_s_" + cfg.name] = round(cfg.ni * cfg.nj * cfg.nk / (t_traj * 1e-3) / 1e6, 1)
         if bc_cpu_in is not None:
             bc_cpu = bc_cpu_baseline(np, bc_cpu_in, layout5, scope)
             bc_cpu_in = None
@@ -476,6 +483,7 @@ def main():
         "lam_flux_targets_interpolated": flux_active,
         "ilmc_beside_step": ilmc,
         "fixer_cpu_baseline": bc_cpu,
+        "traject_cpu_baseline": traj_cpu,
         "interp_gpoints_per_s": round(interp_gpts, 3),
         "interp_alg_GBps": round(alg_interp / (t_interp * 1e-3) / 1e9, 1),
         "fp64_plain_gpoints_per_s": round(plain_gpts, 3),
@@ -490,6 +498,116 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+
+
+def run_fortran(args, np, torch, L, A, S, dev):
+    """The c5 step through the Fortran drop-in layer (gem_amd/fortran/adz_gpu.F90), set up
+    the way GEM's modules hold the state: adz_main() (trajectory; u, v, rhsc on the
+    momentum sets, rhst and rhsf on Adz_pt, extended sources), then adz_tracers(.true.)
+    (q without post-processing, the 60 tracers with lin/min/max and their boundary flux
+    deferred) and adz_tracers(.false.) (set_post_tr_hlt's air masses from the pressures,
+    the flux masses, Bermejo-Conde on all 60), captured in one hipGraph.  The reference
+    keeps lin/min/max for every tracer of the group until the fixer runs, so this needs
+    about 250 GB of HBM at c5."""
+    from gem_amd import adz_fortran as AF
+
+    cfg = S.CONFIGS[args.config]
+    g = S.grid_desc(cfg)
+    ntr = cfg.ntracers
+    t_setup = time.time()
+    layout = (1 - MH, cfg.ni + MH, 1 - MH, cfg.nj + MH)
+    mod_shape = (cfg.nk, cfg.nj + 2 * MH, cfg.ni + 2 * MH)
+    n_mod = math.prod(mod_shape)
+    mod_off = MH + (cfg.ni + 2 * MH) * MH
+    pos_dt = L.ADZ_F32 if args.traj_storage == "f32" else L.ADZ_F64
+    F = AF.FortranStep(g, layout, itraj=cfg.niter, pos_dtype=pos_dt, field_dtype=L.ADZ_F32, math=L.ADZ_MATH_F32)
+    info = F.ctx.info
+    ext_off = (1 - info["lminx"]) + info["nit"] * (1 - info["lminy"])
+    ext_shape = (cfg.nk, cfg.nj + 2 * cfg.halo, cfg.ni + 2 * cfg.halo)
+    w = {k: v.to(dev) for k, v in S.winds(cfg, g, device=dev).items()}
+
+    def smooth(seed, shape, scale, off):
+        f = S.smooth_field(shape, np.random.default_rng(seed), dev)
+        return (off + scale * f / 8.0).reshape(-1).float()
+
+    rhs_src = [smooth(cfg.seed + 31 * q, ext_shape, *sc) for q, sc in
+               enumerate([(20.0, 0.0), (20.0, 0.0), (1.0, 0.0), (15.0, 250.0), (1.0, 0.0)])]
+    rhs_dst = [torch.zeros(n_mod, device=dev) for _ in range(5)]
+    P = [smooth(cfg.seed + 93, mod_shape, 0.002, 0.006)]
+    P += [S.plume_tracer(mod_shape, np.random.default_rng(cfg.seed + 31 * (4 + q)), dev).reshape(-1).float()
+          for q in range(ntr)]
+    M = [torch.zeros(n_mod, device=dev) for _ in P]
+    post = [{k: torch.zeros(n_mod, device=dev) for k in ("lin", "min", "max")} for _ in range(ntr)]
+    rng = np.random.default_rng(cfg.seed + 7)
+    shape2 = mod_shape[1:]
+    ps = torch.from_numpy(rng.uniform(0.97, 1.03, shape2)).to(dev)
+    pm = [(torch.exp(torch.tensor(g["zm"][1:cfg.nk + 2], device=dev))[:, None, None] * ps[None] *
+           (1.0 + 1e-3 * q)).float().reshape(-1) for q in range(2)]
+    pt = (torch.exp(torch.tensor(g["zt"][1:cfg.nk + 1], device=dev))[:, None, None] * ps[None]).float().reshape(-1)
+    p0 = (1e5 * ps).float().reshape(-1)
+    area = ((S.RAYT * g["h"]) ** 2 * torch.ones(shape2, dtype=torch.float64, device=dev)).reshape(-1)
+    mask = torch.ones(shape2[0] * shape2[1], dtype=torch.float64, device=dev)
+    airm1, airm0, pkps = (torch.zeros(n_mod, device=dev) for _ in range(3))
+    F.set_time(cfg.dt)
+    F.set_winds(w)
+    F.set_rhs(rhs_src + [None], rhs_dst + [None], ext_off, mod_off, hydro=True)
+    F.set_tracers(P, M, off=mod_off)
+    F.set_groups(tricub_nt=(1, 1), tricub_wp=(2, ntr))
+    F.set_wp(1, [1] * ntr, [0] * ntr, post, off=mod_off)
+    F.set_bc_fields(airm1, airm0, pkps, mask, mod_off, mod_off)
+    F.set_airmass(pm[0], pm[1], pt, p0, area, mod_off, mod_off)
+    F.set_options(bc_min_max=True)
+    cached = tuned_tiles(args.config, args.tiles)
+    if cached:
+        F.set_tune(0, cached["uvtq"])
+        F.set_tune(1, cached["tracer_groups"])
+    nfields = 5 + 1 + ntr
+    box_u = (info["inu"] - info["i0u"] + 1) * (info["jn"] - info["j0"] + 1) * (cfg.nk - info["k0"] + 1)
+    box_v = (info["in"] - info["i0"] + 1) * (info["jnv"] - info["j0v"] + 1) * (cfg.nk - info["k0"] + 1)
+    box_m = (info["in"] - info["i0"] + 1) * (info["jn"] - info["j0"] + 1) * (cfg.nk - info["k0"] + 1)
+    box_t = (info["in"] - info["i0"] + 1) * (info["jn"] - info["j0"] + 1) * (cfg.nk - info["k0t"] + 1)
+    pts_total = box_u + box_v + box_m + (2 + 1 + ntr) * box_t
+    log(f"fortran layer: {cfg.ni}x{cfg.nj}x{cfg.nk}, {nfields} fields, setup {time.time() - t_setup:.1f}s, "
+        f"{torch.cuda.memory_allocated() / 1e9:.1f} GB allocated")
+    ev = lambda: torch.cuda.Event(enable_timing=True)
+    F.adz_inittraj()
+    for _ in range(max(1, args.warmup)):
+        F.step()  # the first call creates and caches the plans
+    torch.cuda.synchronize()
+    a, b = ev(), ev()
+    a.record()
+    F.step()
+    b.record()
+    torch.cuda.synchronize()
+    eager_ms = a.elapsed_time(b)
+    graph = torch.cuda.CUDAGraph()
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        F.set_stream(s)
+        with torch.cuda.graph(graph, stream=s):
+            F.step()
+    torch.cuda.current_stream().wait_stream(s)
+    F.set_stream(torch.cuda.current_stream())
+    graph.replay()
+    torch.cuda.synchronize()
+    a, b = ev(), ev()
+    a.record()
+    for _ in range(args.steps):
+        graph.replay()
+    b.record()
+    torch.cuda.synchronize()
+    ms_step = a.elapsed_time(b) / args.steps
+    dfc = F.deficits(1, ntr)
+    log(f"fortran layer step: graph {ms_step:.2f} ms, eager {eager_ms:.2f} ms; deficits finite: "
+        f"{all(math.isfinite(x) for x in dfc)}")
+    out = {"fortran_layer": True, "config": cfg.name, "fields": nfields, "points_x_fields_per_step": pts_total,
+           "ms_per_step_graph": round(ms_step, 3), "ms_per_step_eager": round(eager_ms, 3),
+           "gpoints_per_s": round(pts_total / (ms_step * 1e-3) / 1e9, 3),
+           "tiles": cached, "steps": args.steps}
+    print(json.dumps(out), flush=True)
+    del graph
+    F.free()
 
 
 def run_c4(args, np, torch, L, A, S, dev):
@@ -655,17 +773,30 @@ def tuned_tiles(config, path):
         return None
 
 
+def kernel_source_sha():
+    """sha256 (first 16 hex digits) of the sources the batched interpolation kernel is built
+    from: a PMC measurement is used only for the build it was taken on"""
+    import hashlib
+
+    h = hashlib.sha256()
+    for fn in ("adz_batched.hip", "adz_common.h", "adz_eval.h"):
+        with open(os.path.join(ROOT, "gem_amd", "csrc", fn), "rb") as f:
+            h.update(f.read())
+    return h.hexdigest()[:16]
+
+
 def measured_traffic(config, tiles):
     """HBM bytes per launch of the tracer-group kernel from the committed rocprofv3 PMC
     summary (FETCH_SIZE and WRITE_SIZE, each scaled by the same counter's reading for a
-    copy of known size), when it was measured on this workload with these tiles."""
+    copy of known size), when it was measured on this workload with these tiles and a
+    kernel built from the current sources."""
     path = os.path.join(ROOT, "profiles", f"k_plan_{config}_pmc.json")
     try:
         with open(path) as f:
             m = json.load(f)
     except (OSError, ValueError):
         return None
-    if m.get("tiles") != tiles:
+    if m.get("tiles") != tiles or m.get("kernel_source_sha16") != kernel_source_sha():
         return None
     return m.get("hbm_bytes_per_launch")
 
@@ -703,14 +834,49 @@ def ilmc_cpu_baseline(np, inp, gpu_out, layout, box):
             "gpu_bit_exact_vs_cpu": bool(np.array_equal(f.view(np.uint32), gpu_out.view(np.uint32)))}
 
 
+def host_info():
+    """The host's CPU model, the logical CPUs this process may run on, and the threads the
+    CPU baselines use: the process's CPU share (OMP_NUM_THREADS where it is set, as on a
+    one-GPU box) within the CPUs visible to it."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        visible = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        visible = os.cpu_count() or 1
+    share = int(os.environ.get("OMP_NUM_THREADS") or visible)
+    return {"cpu_model": model, "cpus_visible": visible, "threads": max(1, min(share, visible))}
+
+
+def _rate_loop(work, npf, seconds):
+    """passes of work() until `seconds` have passed (at least one): point-fields per second"""
+    reps, t0 = 0, time.time()
+    while reps == 0 or time.time() - t0 < seconds:
+        work()
+        reps += 1
+    return reps * npf / (time.time() - t0), reps
+
+
 def cpu_baseline(np, torch, cfg, g, src, pt, tbox, mod_shape, nf, seconds):
     """The reference CPU interpolator timed on this host on a bounded sample of the same
-    workload: 8 tracers at the thermodynamic departure points of 12 mid-levels.  Uses the
-    reference's turbo_tricublin.c build (oracle/_ref) when present, else the oracle port."""
+    workload: 8 tracers at the thermodynamic departure points of 12 mid-levels,
+    Tricublin_zyx1_m_n per level slice as adz_interp_hlt.F90:104-107 shares them out.
+    Timed with all the process's CPU share and with one core, for the reference's own
+    turbo_tricublin.c (the AVX2 build under oracle/_ref, when present) and for the oracle's
+    restatement of it (the port); `value` is the reference's all-core rate where it ran,
+    else the port's."""
     import concurrent.futures as cf
 
     from oracle import oracle as O
 
+    host = host_info()
     i0, in_, j0, jn, k0, nk = tbox
     nx, ny = in_ - i0 + 1, jn - j0 + 1
     klo, khi = 36, 47                      # target levels of the sample
@@ -724,40 +890,89 @@ def cpu_baseline(np, torch, cfg, g, src, pt, tbox, mod_shape, nf, seconds):
     levels = g["zt"][plo:phi + 1]
     ni_m, nj_m = mod_shape[2], mod_shape[1]
     base = MH + ni_m * MH
-    nthreads = max(1, min(16, os.cpu_count() or 1))
+    nthreads = host["threads"]
     slices = [np.ascontiguousarray(pos[k].reshape(-1)) for k in range(pos.shape[0])]
-    npts = sum(s.size // 3 for s in slices)
+    npf = sum(s.size // 3 for s in slices) * nfs
+    legs = {}
     try:
         R = O.RefKernel("avx2")
         lv = R.setup(levels, ni_m, nj_m, 0, 0)
-        kind = "reference"
-
-        def work(s):
-            return R.zyx1_m_n(lv, fields, base, s)
     except (FileNotFoundError, OSError):
-        R, kind = None, "port"
-        zt = O.Ztab(levels, ni_m, nj_m, 0, 0)
+        R = None
+    if R is not None:
+        for nth in (nthreads, 1):
+            with cf.ThreadPoolExecutor(nth) as ex:  # one work item per level slice, as the OpenMP do
+                rate, reps = _rate_loop(lambda: list(ex.map(lambda s: R.zyx1_m_n(lv, fields, base, s), slices)), npf,
+                                        seconds * 0.3)
+            legs[f"reference_{'all' if nth == nthreads else 'one'}"] = rate / 1e9
+    zt = O.Ztab(levels, ni_m, nj_m, 0, 0)
+    allpos = np.concatenate(slices)
+    for nth in (nthreads, 1):
+        O.set_threads(nth)
+        rate, reps = _rate_loop(lambda: O.zyx1_m_n(zt, fields, base, allpos), npf, seconds * 0.2)
+        legs[f"port_{'all' if nth == nthreads else 'one'}"] = rate / 1e9
+    O.set_threads(nthreads)
+    kind = "reference" if R is not None else "port"
+    value = legs[f"{kind}_all"]
+    log(f"cpu baseline: {', '.join(f'{k} {v:.3f}' for k, v in legs.items())} Gpt/s; {host}")
+    out = {"value": round(value, 4), "unit": "Gpoints/s", "cores": nthreads, "kind": kind,
+           "single_core": round(legs[f"{kind}_one"], 4),
+           "legs_gpoints_per_s": {k: round(v, 4) for k, v in legs.items()},
+           "host": host,
+           "sample": f"{nfs} tracers x {npf // nfs} departure points (levels {klo}-{khi} of {cfg.name}), "
+                     f"Tricublin_zyx1_m_n per level slice, FP32 fields; each leg timed for about "
+                     f"{seconds * (0.3 if kind == 'reference' else 0.2):.1f} s"}
+    if R is not None:
+        out["port_over_reference"] = round(legs["port_all"] / legs["reference_all"], 3)
+    return out
 
-        def work(s):
-            return O.zyx1_m_n(zt, fields, base, s)
-    reps, t0 = 0, time.time()
-    if kind == "reference":
-        with cf.ThreadPoolExecutor(nthreads) as ex:  # one work item per level slice, as the OpenMP do
-            while time.time() - t0 < seconds:
-                list(ex.map(work, slices))
-                reps += 1
-    else:
-        os.environ.setdefault("OMP_NUM_THREADS", str(nthreads))
-        allpos = np.concatenate(slices)
-        while time.time() - t0 < seconds:
-            work(allpos)
+
+def traject_baseline(np, torch, L, A, S, dev):
+    """The trajectory of c1 (200x150x60, the CPU-runnable workload): the oracle's
+    restatement of adz_traject (FP64 iteration, the reference's FP32 storage) over the
+    whole tile on all the process's CPU share and on one core, and the GPU's adz_traject on
+    the same winds; departure points (all levels, every position set) per second of a
+    step after the first (3 iterations)."""
+    from oracle import oracle as O
+
+    host = host_info()
+    cfg = S.CONFIGS["c1"]
+    g = S.grid_desc(cfg)
+    w = S.winds(cfg, g)
+    wn = {k: v.numpy() for k, v in w.items()}
+    G = O.Grid(g)
+    first = G.traject(wn, cfg.dt, storage="f32", niter=cfg.niter, want=["pt"])
+    state = dict(pxyzm=first["pxyzm"], wpxyz=first["wpxyz"])
+    npts = cfg.ni * cfg.nj * cfg.nk
+    rates = {}
+    for nth in (host["threads"], 1):
+        O.set_threads(nth)
+        t0, reps = time.time(), 0
+        while reps == 0 or time.time() - t0 < 2.0:
+            G.traject(wn, cfg.dt, storage="f32", state=state, niter=cfg.niter, want=["pt"])
             reps += 1
-    dt = time.time() - t0
-    rate = reps * npts * nfs / dt / 1e9
-    log(f"cpu baseline ({kind}, {nthreads} threads): {rate:.3f} Gpt/s")
-    return {"value": round(rate, 4), "unit": "Gpoints/s", "cores": nthreads, "kind": kind,
-            "sample": f"{nfs} tracers x {npts} departure points (levels {klo}-{khi} of {cfg.name}), "
-                      f"Tricublin_zyx1_m_n per level slice, FP32 fields, {reps} passes in {dt:.1f} s"}
+        rates["all" if nth == host["threads"] else "one"] = reps * npts / (time.time() - t0)
+    O.set_threads(host["threads"])
+    ctx = A.AdzContext(g, L.ADZ_F32, itraj=cfg.niter)
+    wd = {k: v.to(dev) for k, v in w.items()}
+    ctx.adz_inittraj()
+    run = lambda: ctx.adz_traject(cfg.dt, wd["uu_ext"], wd["vv_ext"], wd["ww_ext"], wd["ut0"], wd["vt0"], wd["zdt0"])
+    run()
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(20):
+        run()
+    b.record()
+    torch.cuda.synchronize()
+    gpu_rate = 20 * npts / (a.elapsed_time(b) * 1e-3)
+    ctx.free()
+    log(f"trajectory c1: CPU port {rates['all'] / 1e6:.1f} Mpts/s ({host['threads']} threads), "
+        f"{rates['one'] / 1e6:.2f} Mpts/s (1 thread); GPU {gpu_rate / 1e6:.0f} Mpts/s")
+    return {"workload": "c1 200x150x60, 3 iterations, FP32 position storage", "kind": "port",
+            "cpu_mpoints_per_s": round(rates["all"] / 1e6, 3), "cores": host["threads"],
+            "cpu_mpoints_per_s_one_core": round(rates["one"] / 1e6, 3),
+            "gpu_mpoints_per_s": round(gpu_rate / 1e6, 1)}
 
 
 if __name__ == "__main__":

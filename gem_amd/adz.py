@@ -123,10 +123,23 @@ class AdzContext:
         buf = np.ascontiguousarray(buf, dtype=np.uint8)
         L.check(L.lib().adz_ctx_load_state(self.h, buf.ctypes.data, buf.size, _stream()), "adz_ctx_load_state")
 
+    @classmethod
+    def view(cls, h, grid, pos_dtype):
+        """A context another owner created (e.g. the Fortran layer's): arrays and tables can be
+        read through it; it is never freed from here."""
+        v = cls.__new__(cls)
+        v.grid, v.pos_dtype, v.h, v._owner = grid, pos_dtype, C.c_void_p(h), False
+        iv = np.zeros(24, np.int32)
+        dv = np.zeros(9)
+        L.check(L.lib().adz_ctx_info(v.h, iv.ctypes.data, dv.ctypes.data), "adz_ctx_info")
+        v.info = dict(zip(INFO_INTS, iv.tolist()))
+        v.info.update(zip(INFO_DBLS, dv.tolist()))
+        return v
+
     def free(self):
-        if getattr(self, "h", None) is not None and self.h.value:
+        if getattr(self, "_owner", True) and getattr(self, "h", None) is not None and self.h.value:
             L.lib().adz_ctx_free(self.h)
-            self.h = None
+        self.h = None
 
     def __del__(self):
         try:

@@ -4,8 +4,11 @@
 !   adz_set -> adz_inittraj -> adz_traject(dt) -> adz_tricub_hlt(tracers at Adz_pt, with
 !   Adz_post, with the LAM boundary flux) -> adz_post_tr(1) -> adz_post_tr(0) (Bermejo-Conde)
 !
-! Usage: adz_f_driver <input file> <output file> [qv|zlf]
+! Usage: adz_f_driver <input file> <output file> [qv|zlf|graph]
 !        (qv: the tracers with quintic vertical interpolation, F_QV_L = .true.;
+!         graph: the tricubic step run once, then from the same start again as a hipGraph
+!         captured on a created stream (hipStreamBeginCapture .. hipGraphLaunch): the
+!         output is the replay's;
 !         zlf: the ZLF boundary treatment, Adz_BC_LAM_flux = 2, as adz_tracers.F90:88-99,
 !         154-163 runs it: interpolation on Adz_pb, the piloting values of the
 !         destinations saved before and restored after the fixer)
@@ -27,7 +30,9 @@ program adz_f_driver
    use, intrinsic :: iso_c_binding
    use, intrinsic :: iso_fortran_env, only: REAL64, error_unit
    use adz_hip, only: adz_grid, ADZ_F32, ADZ_F64, hipMalloc, hipMemcpy, hipFree, hipDeviceSynchronize, &
-                      hipMemcpyHostToDevice, hipMemcpyDeviceToHost
+                      hipMemcpyHostToDevice, hipMemcpyDeviceToHost, hipStreamCreate, hipStreamDestroy, &
+                      hipStreamBeginCapture, hipStreamEndCapture, hipGraphInstantiate, hipGraphLaunch, &
+                      hipGraphExecDestroy, hipGraphDestroy, hipStreamSynchronize
    use adz_gpu
    implicit none
    character(len=512) :: fin, fout, mode
@@ -43,6 +48,7 @@ program adz_f_driver
    type(adz_grid), target :: g
    type(c_ptr), allocatable :: d_src(:), d_dst(:), d_lin(:), d_min(:), d_max(:), d_pil(:)
    real(kind=REAL64), allocatable :: deficits(:)
+   type(c_ptr) :: graph, gexec
 
    call get_command_argument(1, fin)
    call get_command_argument(2, fout)
@@ -57,7 +63,8 @@ program adz_f_driver
          allocate (Adz_stack(1))
          call adz_tricub_hlt(Adz_stack, 1, c_null_ptr, c_null_ptr, 1, 1, 1, 1, 1, 1, F_ext_L=.false., F_QV_L=.true.)
       else
-         allocate (Adz_stack(1), Adz_post(1))
+         allocate (Adz_stack(1), Adz_post_3CWP(1))
+         Adz_post => Adz_post_3CWP
          call ILMC_LAM(1, 'TR/X:M', 1, 1, 1, 1)
       end if
       write (*, '(a)') 'not refused'
@@ -93,7 +100,8 @@ program adz_f_driver
    ut0 = upload_f32(n_ext); vt0 = upload_f32(n_ext); zdt0 = upload_f32(n_ext)
 
    allocate (d_src(nfld), d_dst(nfld), d_lin(nfld), d_min(nfld), d_max(nfld), d_pil(nfld))
-   allocate (Adz_stack(nfld), Adz_post(nfld), Tr_3CWP(nfld), mass(nfld), mono(nfld), deficits(nfld))
+   allocate (Adz_stack(nfld), Adz_post_3CWP(nfld), Tr_3CWP(nfld), mass(nfld), mono(nfld), deficits(nfld))
+   Adz_post => Adz_post_3CWP
    do n = 1, nfld
       d_src(n) = upload_f32(n_mod)
       d_dst(n) = zeros_f32(n_mod); d_lin(n) = zeros_f32(n_mod)
@@ -120,20 +128,29 @@ program adz_f_driver
    airm1 = offset(airm1); airm0 = offset(airm0); pkps = offset(pkps)
    geomh_mask_8 = c_ptr_add(geomh_mask_8, 8_c_size_t * int(mh + (ni + 2 * mh) * mh, c_size_t))
 
-   call adz_inittraj()
-   call adz_traject(dbl(8))
-   if (trim(mode) == 'zlf') then
-      call adz_BC_LAM_zlf_0_hlt(nfld, 0)
-      call adz_tricub_hlt(Adz_stack, nfld, Adz_pb, Adz_cpntr_t, Adz_num_b, Adz_i0b, Adz_inb, Adz_j0b, Adz_jnb, &
-                          1, .false., Tr_3CWP)
-      call adz_BC_LAM_zlf_0_hlt(nfld, 1)
-   else
-      call adz_tricub_hlt(Adz_stack, nfld, Adz_pt, Adz_cpntr_t, Adz_num_t, Adz_i0, Adz_in, Adz_j0, Adz_jn, &
-                          Adz_k0t, .false., Tr_3CWP, F_QV_L=trim(mode) == 'qv')
+   if (trim(mode) == 'graph') then
+      if (hipStreamCreate(Adz_stream) /= 0) error stop 'hipStreamCreate'
    end if
-   call adz_post_tr(1)
-   call adz_post_tr(0)
-   if (trim(mode) == 'zlf') call adz_BC_LAM_zlf_0_hlt(nfld, 2)
+   call adz_inittraj()
+   call one_step()
+   if (trim(mode) == 'graph') then
+      ! the same step again from the same start: the trajectory and the carried deficits
+      ! restart, then the step is captured once and the graph launched
+      if (hipStreamSynchronize(Adz_stream) /= 0) error stop 'hipStreamSynchronize'
+      call adz_inittraj()
+      do n = 1, nfld
+         Tr_3CWP(n)%BC_mass_deficit = 0.d0
+      end do
+      call adz_bc_upload_deficits()
+      if (hipStreamBeginCapture(Adz_stream, 0_c_int) /= 0) error stop 'hipStreamBeginCapture'
+      call one_step()
+      if (hipStreamEndCapture(Adz_stream, graph) /= 0) error stop 'hipStreamEndCapture'
+      if (hipGraphInstantiate(gexec, graph, c_null_ptr, c_null_ptr, 0_c_size_t) /= 0) error stop 'hipGraphInstantiate'
+      if (hipGraphLaunch(gexec, Adz_stream) /= 0) error stop 'hipGraphLaunch'
+      if (hipStreamSynchronize(Adz_stream) /= 0) error stop 'hipStreamSynchronize'
+      if (hipGraphExecDestroy(gexec) /= 0 .or. hipGraphDestroy(graph) /= 0) error stop 'hipGraphDestroy'
+   end if
+   call adz_bc_sync_deficits()
    if (hipDeviceSynchronize() /= 0) error stop 'hipDeviceSynchronize'
 
    open (newunit=u, file=trim(fout), access='stream', form='unformatted', status='replace')
@@ -161,13 +178,34 @@ program adz_f_driver
    write (u) deficits
    close (u)
    call adz_gpu_finalize()
+   if (trim(mode) == 'graph') then
+      if (hipStreamDestroy(Adz_stream) /= 0) error stop 'hipStreamDestroy'
+   end if
    write (*, '(a,i0,a)') 'adz_f_driver: ', nfld, ' tracers advected and fixed'
 
 contains
 
+   ! trajectory, interpolation of the tracers at their positions with Adz_post (and the
+   ! boundary flux), adz_post_tr(1) (ILMC, registry) and adz_post_tr(0) (the fixer)
+   subroutine one_step()
+      call adz_traject(dbl(8))
+      if (trim(mode) == 'zlf') then
+         call adz_BC_LAM_zlf_0_hlt(nfld, 0)
+         call adz_tricub_hlt(Adz_stack, nfld, Adz_pb, Adz_cpntr_t, Adz_num_b, Adz_i0b, Adz_inb, Adz_j0b, Adz_jnb, &
+                             1, .false., Tr_3CWP)
+         call adz_BC_LAM_zlf_0_hlt(nfld, 1)
+      else
+         call adz_tricub_hlt(Adz_stack, nfld, Adz_pt, Adz_cpntr_t, Adz_num_t, Adz_i0, Adz_in, Adz_j0, Adz_jn, &
+                             Adz_k0t, .false., Tr_3CWP, F_QV_L=trim(mode) == 'qv')
+      end if
+      call adz_post_tr(1)
+      call adz_post_tr(0)
+      if (trim(mode) == 'zlf') call adz_BC_LAM_zlf_0_hlt(nfld, 2)
+   end subroutine one_step
+
    subroutine print_sizes()
       use adz_hip, only: adz_field, adz_item, adz_tune, adz_bc_tracer, adz_bc_domain, adz_flux_tracer, &
-                         adz_flux_domain
+                         adz_flux_domain, adz_airmass_fields
       type(adz_field) :: f
       type(adz_item) :: it
       type(adz_tune) :: t
@@ -175,8 +213,9 @@ contains
       type(adz_bc_domain) :: d
       type(adz_flux_tracer) :: ft
       type(adz_flux_domain) :: fd
-      write (*, '(a,8(1x,i0))') 'sizes', c_sizeof(g), c_sizeof(f), c_sizeof(it), c_sizeof(t), c_sizeof(b), &
-         c_sizeof(d), c_sizeof(ft), c_sizeof(fd)
+      type(adz_airmass_fields) :: am
+      write (*, '(a,9(1x,i0))') 'sizes', c_sizeof(g), c_sizeof(f), c_sizeof(it), c_sizeof(t), c_sizeof(b), &
+         c_sizeof(d), c_sizeof(ft), c_sizeof(fd), c_sizeof(am)
    end subroutine print_sizes
 
    function c_ptr_add(p, bytes) result(q)

@@ -1,0 +1,218 @@
+! adz_gpu_capi.F90 -- C-callable entry points into the Fortran drop-in layer (module
+! adz_gpu), so that a host program in another language can set the module state the way
+! GEM's own modules hold it and run the reference's step routines on it: adz_main() and
+! adz_tracers(F_before_psadj_L) as dynstep.F90:93-98 calls them.
+!
+! These routines only copy arguments into the module variables of adz_gpu (device
+! addresses, counts, codes) and call its routines; they have no reference counterpart.
+! Python's gem_amd.adz_fortran uses them to run and time the Fortran layer on the same
+! device arrays as the C ABI.  Errors stop the process with a message, as the Fortran
+! layer does (gem_error).
+module adz_gpu_capi
+   use, intrinsic :: iso_c_binding
+   use, intrinsic :: iso_fortran_env, only: REAL64
+   use adz_hip, only: adz_grid, adz_tune, ADZ_F32, ADZ_F64
+   use adz_gpu
+   implicit none
+
+contains
+
+   ! adz_set on one tile with the given storage of positions and fields and the arithmetic
+   ! of the interpolation (ADZ_MATH_REF or ADZ_MATH_F32)
+   subroutine adz_f_set(g, itraj, layout, pos_dtype, field_dtype, math) bind(C, name="adz_f_set")
+      type(adz_grid), intent(in) :: g
+      integer(c_int), value :: itraj, pos_dtype, field_dtype, math
+      integer(c_int), intent(in) :: layout(4)
+      Adz_pos_dtype = pos_dtype
+      Adz_field_dtype = field_dtype
+      Adz_interp_math = math
+      call adz_gpu_set(g, int(itraj), int(layout))
+   end subroutine adz_f_set
+
+   subroutine adz_f_finalize() bind(C, name="adz_f_finalize")
+      call adz_gpu_finalize()
+      if (allocated(tracers_P)) deallocate (tracers_P, tracers_M, tracers_B)
+      if (allocated(Tr_3CWP)) deallocate (Tr_3CWP)
+      if (allocated(Tr_BQWP)) deallocate (Tr_BQWP)
+      if (allocated(Adz_post_3CWP)) deallocate (Adz_post_3CWP)
+      if (allocated(Adz_post_BQWP)) deallocate (Adz_post_BQWP)
+      if (allocated(Adz_bc)) deallocate (Adz_bc)
+      if (allocated(Adz_stack)) deallocate (Adz_stack)
+      nullify (Adz_post)
+   end subroutine adz_f_finalize
+
+   subroutine adz_f_set_stream(stream) bind(C, name="adz_f_set_stream")
+      type(c_ptr), value :: stream
+      Adz_stream = stream
+   end subroutine adz_f_set_stream
+
+   ! time step and off-centring (cstv), advection option (Schm_advec)
+   subroutine adz_f_set_time(dt, bA_m, bA, advec) bind(C, name="adz_f_set_time")
+      real(c_double), value :: dt, bA_m, bA
+      integer(c_int), value :: advec
+      Cstv_dt_8 = dt; Cstv_bA_m_8 = bA_m; Cstv_bA_8 = bA; Schm_advec = advec
+   end subroutine adz_f_set_time
+
+   subroutine adz_f_set_winds(uu, vv, ww, u0, v0, zd0) bind(C, name="adz_f_set_winds")
+      type(c_ptr), value :: uu, vv, ww, u0, v0, zd0
+      Adz_uu_ext = uu; Adz_vv_ext = vv; Adz_ww_ext = ww; ut0 = u0; vt0 = v0; zdt0 = zd0
+   end subroutine adz_f_set_winds
+
+   ! orhs{u,v,c,t,f,w}_ext (extended layout) and rhs{u,v,c,t,f,w} (model layout)
+   subroutine adz_f_set_rhs(src, dst, hydro, hauteur) bind(C, name="adz_f_set_rhs")
+      type(c_ptr), intent(in) :: src(6), dst(6)
+      integer(c_int), value :: hydro, hauteur
+      orhsu_ext = src(1); orhsv_ext = src(2); orhsc_ext = src(3)
+      orhst_ext = src(4); orhsf_ext = src(5); orhsw_ext = src(6)
+      rhsu = dst(1); rhsv = dst(2); rhsc = dst(3); rhst = dst(4); rhsf = dst(5); rhsw = dst(6)
+      Dynamics_hydro_L = hydro /= 0
+      Dynamics_hauteur_L = hauteur /= 0
+   end subroutine adz_f_set_rhs
+
+   ! tracers_P / tracers_M / tracers_B (may be null) of ntr tracers
+   subroutine adz_f_set_tracers(ntr, p, m, b) bind(C, name="adz_f_set_tracers")
+      integer(c_int), value :: ntr
+      type(c_ptr), intent(in) :: p(ntr), m(ntr), b(ntr)
+      if (allocated(tracers_P)) deallocate (tracers_P, tracers_M, tracers_B)
+      allocate (tracers_P(ntr), tracers_M(ntr), tracers_B(ntr))
+      tracers_P = p; tracers_M = m; tracers_B = b
+   end subroutine adz_f_set_tracers
+
+   ! the tracer groups of tr3d: (deb, ntr) of TRICUB_NT, BICHQV_NT, TRICUB_WP, BICHQV_WP
+   subroutine adz_f_set_groups(deb, ntr) bind(C, name="adz_f_set_groups")
+      integer(c_int), intent(in) :: deb(4), ntr(4)
+      Tr3d_debTRICUB_NT = deb(1); Tr3d_ntrTRICUB_NT = ntr(1)
+      Tr3d_debBICHQV_NT = deb(2); Tr3d_ntrBICHQV_NT = ntr(2)
+      Tr3d_debTRICUB_WP = deb(3); Tr3d_ntrTRICUB_WP = ntr(3)
+      Tr3d_debBICHQV_WP = deb(4); Tr3d_ntrBICHQV_WP = ntr(4)
+   end subroutine adz_f_set_groups
+
+   ! the metas (mass code, mono code, carried deficit) and Adz_post arrays (lin, min, max)
+   ! of the group with post-processing: which = 1 (Tr_3CWP, Adz_post_3CWP) or 2 (Tr_BQWP)
+   subroutine adz_f_set_wp(which, n, mass, mono, deficit, lin, mn, mx) bind(C, name="adz_f_set_wp")
+      integer(c_int), value :: which, n
+      integer(c_int), intent(in) :: mass(n), mono(n)
+      real(c_double), intent(in) :: deficit(n)
+      type(c_ptr), intent(in) :: lin(n), mn(n), mx(n)
+      type(meta_tracers), allocatable :: t(:)
+      type(post), allocatable :: q(:)
+      integer :: k
+      allocate (t(n), q(n))
+      do k = 1, n
+         t(k)%mass = mass(k); t(k)%mono = mono(k); t(k)%BC_mass_deficit = deficit(k)
+         write (t(k)%name, '(a1,i3.3)') merge('C', 'Q', which == 1), k
+         q(k) = post(lin(k), mn(k), mx(k))
+      end do
+      if (which == 1) then
+         call move_alloc(t, Tr_3CWP)
+         call move_alloc(q, Adz_post_3CWP)
+      else
+         call move_alloc(t, Tr_BQWP)
+         call move_alloc(q, Adz_post_BQWP)
+      end if
+      nullify (Adz_post)
+      Adz_bc_deficit_upload_L = .true.
+   end subroutine adz_f_set_wp
+
+   ! the fixer's inputs: air masses at P and M, pr_t/pr_s, geomh_mask_8
+   subroutine adz_f_set_bc_fields(a1, a0, pk, mask) bind(C, name="adz_f_set_bc_fields")
+      type(c_ptr), value :: a1, a0, pk, mask
+      airm1 = a1; airm0 = a0; pkps = pk; geomh_mask_8 = mask
+   end subroutine adz_f_set_bc_fields
+
+   ! the inputs of set_post_tr_hlt: pw_pm_plus, pw_pm_moins, pw_pt_moins, pw_p0_moins,
+   ! geomh_area_8, the humidity's tracer index (0: none) and the dry-mixing-ratio flag
+   subroutine adz_f_set_airmass(pm_p, pm_m, pt_m, p0_m, area, hu, dry) bind(C, name="adz_f_set_airmass")
+      type(c_ptr), value :: pm_p, pm_m, pt_m, p0_m, area
+      integer(c_int), value :: hu, dry
+      pw_pm_plus = pm_p; pw_pm_moins = pm_m; pw_pt_moins = pt_m; pw_p0_moins = p0_m; geomh_area_8 = area
+      Tr3d_hu = hu
+      Schm_dry_mixing_ratio_L = dry /= 0
+   end subroutine adz_f_set_airmass
+
+   ! options: Adz_BC_min_max_L, Adz_ILMC_sweep_max, Adz_ILMC_min_max_L, Schm_autobar_L
+   subroutine adz_f_set_options(bc_min_max, ilmc_sweep_max, ilmc_min_max, autobar) bind(C, name="adz_f_set_options")
+      integer(c_int), value :: bc_min_max, ilmc_sweep_max, ilmc_min_max, autobar
+      Adz_BC_min_max_L = bc_min_max /= 0
+      Adz_ILMC_sweep_max = ilmc_sweep_max
+      Adz_ILMC_min_max_L = ilmc_min_max /= 0
+      Schm_autobar_L = autobar /= 0
+   end subroutine adz_f_set_options
+
+   ! tile parameters of the batched launches: which = 0 adz_main's, 1 the tracer groups'
+   subroutine adz_f_set_tune(which, t) bind(C, name="adz_f_set_tune")
+      integer(c_int), value :: which
+      type(adz_tune), intent(in) :: t
+      if (which == 0) then
+         Adz_tune_main = t
+      else
+         Adz_tune_tracers = t
+      end if
+   end subroutine adz_f_set_tune
+
+   subroutine adz_f_inittraj() bind(C, name="adz_f_inittraj")
+      call adz_inittraj()
+   end subroutine adz_f_inittraj
+
+   subroutine adz_f_main() bind(C, name="adz_f_main")
+      call adz_main()
+   end subroutine adz_f_main
+
+   subroutine adz_f_tracers(before_psadj) bind(C, name="adz_f_tracers")
+      integer(c_int), value :: before_psadj
+      call adz_tracers(before_psadj /= 0)
+   end subroutine adz_f_tracers
+
+   ! adz_main, then adz_tracers before and after the pressure adjustment (dynstep.F90:93-98,
+   ! with no adjustment in between)
+   subroutine adz_f_step() bind(C, name="adz_f_step")
+      call adz_main()
+      call adz_tracers(.true.)
+      call adz_tracers(.false.)
+   end subroutine adz_f_step
+
+   ! the carried deficits of the tracers with post-processing, group `which`, into out(n)
+   subroutine adz_f_deficits(which, n, out) bind(C, name="adz_f_deficits")
+      integer(c_int), value :: which, n
+      real(c_double), intent(out) :: out(n)
+      integer :: k
+      call adz_bc_sync_deficits()
+      out = 0.d0
+      do k = 1, n
+         if (which == 1) then
+            if (allocated(Tr_3CWP)) out(k) = Tr_3CWP(k)%BC_mass_deficit
+         else
+            if (allocated(Tr_BQWP)) out(k) = Tr_BQWP(k)%BC_mass_deficit
+         end if
+      end do
+   end subroutine adz_f_deficits
+
+   ! the context of the module (adz_ctx*, for reading its position sets through the C ABI)
+   function adz_f_ctx() bind(C, name="adz_f_ctx") result(c)
+      type(c_ptr) :: c
+      c = Adz_ctx
+   end function adz_f_ctx
+
+   ! new carried deficits of group `which` (a restart, or to start again): written into the
+   ! metas and uploaded at once (synchronous: call outside a capture)
+   subroutine adz_f_set_deficits(which, n, values) bind(C, name="adz_f_set_deficits")
+      integer(c_int), value :: which, n
+      real(c_double), intent(in) :: values(n)
+      integer :: k
+      do k = 1, n
+         if (which == 1) then
+            Tr_3CWP(k)%BC_mass_deficit = values(k)
+         else
+            Tr_BQWP(k)%BC_mass_deficit = values(k)
+         end if
+      end do
+      call adz_bc_upload_deficits()
+   end subroutine adz_f_set_deficits
+
+   ! the position set arrays and counts of the module: Adz_pm, Adz_pmu, Adz_pmv, Adz_pt, Adz_pb
+   subroutine adz_f_info(ints) bind(C, name="adz_f_info")
+      integer(c_int), intent(out) :: ints(8)
+      ints = [Adz_num_u, Adz_num_v, Adz_num_q, Adz_num_t, Adz_num_b, Adz_k0, Adz_k0t, merge(1, 0, Adz_BC_LAM_zlf_L)]
+   end subroutine adz_f_info
+
+end module adz_gpu_capi

@@ -88,6 +88,15 @@ module adz_hip
       integer(c_int) :: ic0, ic1, jc0, jc1, kc0
    end type adz_flux_domain
 
+   type, bind(C) :: adz_airmass_fields
+      integer(c_int) :: minx = 0, maxx = 0, miny = 0, maxy = 0
+      type(c_ptr)    :: pm_p = c_null_ptr, pm_m = c_null_ptr, hu_p = c_null_ptr, hu_m = c_null_ptr
+      type(c_ptr)    :: hyd_p = c_null_ptr, hyd_m = c_null_ptr  ! HOST arrays of nhyd device addresses
+      integer(c_int) :: nhyd = 0
+      type(c_ptr)    :: pt_m = c_null_ptr, p0_m = c_null_ptr, area = c_null_ptr
+      integer(c_int) :: dry_mixing_ratio = 0, wload = 0
+   end type adz_airmass_fields
+
    interface
       function adz_last_error() bind(C, name="adz_last_error") result(msg)
          import :: c_ptr
@@ -354,6 +363,23 @@ module adz_hip
          type(c_ptr), value :: fx
          integer(c_int) :: rc
       end function
+      function adz_set_post_tr(ctx, f, grav, airm1, airm0, pkps, dtype, stream) bind(C, name="adz_set_post_tr") &
+         result(rc)
+         import :: c_int, c_ptr, c_double, adz_airmass_fields
+         type(c_ptr), value :: ctx
+         type(adz_airmass_fields), intent(in) :: f
+         real(c_double), value :: grav
+         type(c_ptr), value :: airm1, airm0, pkps
+         integer(c_int), value :: dtype
+         type(c_ptr), value :: stream
+         integer(c_int) :: rc
+      end function
+      function adz_plan_status(plan, aborted) bind(C, name="adz_plan_status") result(rc)
+         import :: c_int, c_ptr
+         type(c_ptr), value :: plan
+         integer(c_int), intent(out) :: aborted
+         integer(c_int) :: rc
+      end function
    end interface
 
    ! The few HIP runtime calls the Fortran layer needs (libamdhip64).
@@ -379,6 +405,91 @@ module adz_hip
       end function
       function hipDeviceSynchronize() bind(C, name="hipDeviceSynchronize") result(rc)
          import :: c_int
+         integer(c_int) :: rc
+      end function
+      function hipMemcpyAsync(dst, src, bytes, kind, stream) bind(C, name="hipMemcpyAsync") result(rc)
+         import :: c_int, c_size_t, c_ptr
+         type(c_ptr), value :: dst, src
+         integer(c_size_t), value :: bytes
+         integer(c_int), value :: kind
+         type(c_ptr), value :: stream
+         integer(c_int) :: rc
+      end function
+      function hipStreamSynchronize(stream) bind(C, name="hipStreamSynchronize") result(rc)
+         import :: c_int, c_ptr
+         type(c_ptr), value :: stream
+         integer(c_int) :: rc
+      end function
+      function hipStreamCreate(stream) bind(C, name="hipStreamCreate") result(rc)
+         import :: c_int, c_ptr
+         type(c_ptr), intent(out) :: stream
+         integer(c_int) :: rc
+      end function
+      function hipStreamDestroy(stream) bind(C, name="hipStreamDestroy") result(rc)
+         import :: c_int, c_ptr
+         type(c_ptr), value :: stream
+         integer(c_int) :: rc
+      end function
+      ! graph capture of the work enqueued on a stream (mode 0: global, the strictest)
+      function hipStreamBeginCapture(stream, mode) bind(C, name="hipStreamBeginCapture") result(rc)
+         import :: c_int, c_ptr
+         type(c_ptr), value :: stream
+         integer(c_int), value :: mode
+         integer(c_int) :: rc
+      end function
+      function hipStreamEndCapture(stream, graph) bind(C, name="hipStreamEndCapture") result(rc)
+         import :: c_int, c_ptr
+         type(c_ptr), value :: stream
+         type(c_ptr), intent(out) :: graph
+         integer(c_int) :: rc
+      end function
+      function hipGraphInstantiate(exec, graph, errnode, logbuf, bufsize) bind(C, name="hipGraphInstantiate") &
+         result(rc)
+         import :: c_int, c_ptr, c_size_t
+         type(c_ptr), intent(out) :: exec
+         type(c_ptr), value :: graph, errnode, logbuf
+         integer(c_size_t), value :: bufsize
+         integer(c_int) :: rc
+      end function
+      function hipGraphLaunch(exec, stream) bind(C, name="hipGraphLaunch") result(rc)
+         import :: c_int, c_ptr
+         type(c_ptr), value :: exec, stream
+         integer(c_int) :: rc
+      end function
+      function hipGraphExecDestroy(exec) bind(C, name="hipGraphExecDestroy") result(rc)
+         import :: c_int, c_ptr
+         type(c_ptr), value :: exec
+         integer(c_int) :: rc
+      end function
+      function hipGraphDestroy(graph) bind(C, name="hipGraphDestroy") result(rc)
+         import :: c_int, c_ptr
+         type(c_ptr), value :: graph
+         integer(c_int) :: rc
+      end function
+      function hipEventCreate(ev) bind(C, name="hipEventCreate") result(rc)
+         import :: c_int, c_ptr
+         type(c_ptr), intent(out) :: ev
+         integer(c_int) :: rc
+      end function
+      function hipEventRecord(ev, stream) bind(C, name="hipEventRecord") result(rc)
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ev, stream
+         integer(c_int) :: rc
+      end function
+      function hipEventSynchronize(ev) bind(C, name="hipEventSynchronize") result(rc)
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ev
+         integer(c_int) :: rc
+      end function
+      function hipEventElapsedTime(ms, ev0, ev1) bind(C, name="hipEventElapsedTime") result(rc)
+         import :: c_int, c_ptr, c_float
+         real(c_float), intent(out) :: ms
+         type(c_ptr), value :: ev0, ev1
+         integer(c_int) :: rc
+      end function
+      function hipEventDestroy(ev) bind(C, name="hipEventDestroy") result(rc)
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ev
          integer(c_int) :: rc
       end function
    end interface

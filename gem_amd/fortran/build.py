@@ -1,7 +1,9 @@
 """Build recipe for the Fortran layer: the ISO_C_BINDING module (adz_hip.F90), the drop-in
-module with the reference's adz_* signatures (adz_gpu.F90), and the host driver
-(adz_f_driver.F90) linked against libadz_hip.so.  In-tree outputs:
-gem_amd/fortran/_build/*.o, *.mod and gem_amd/fortran/adz_f_driver.
+module with the reference's adz_* signatures (adz_gpu.F90), its C-callable entry points
+(adz_gpu_capi.F90), the shared library of the three (libadz_gpu.so, used from Python by
+gem_amd.adz_fortran) and the host driver (adz_f_driver.F90), all linked against
+libadz_hip.so.  In-tree outputs: gem_amd/fortran/_build/*.o, *.mod,
+gem_amd/fortran/libadz_gpu.so and gem_amd/fortran/adz_f_driver.
 
     python gem_amd/fortran/build.py
 """
@@ -14,7 +16,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 PKG = os.path.dirname(HERE)
 OUT = os.path.join(HERE, "_build")
 DRIVER = os.path.join(HERE, "adz_f_driver")
-MODULES = ["adz_hip.F90", "adz_gpu.F90"]
+SHLIB = os.path.join(HERE, "libadz_gpu.so")
+MODULES = ["adz_hip.F90", "adz_gpu.F90", "adz_gpu_capi.F90"]
 
 
 def compiler():
@@ -31,7 +34,7 @@ def build(verbose=False):
         print("fortran: no amdflang/flang on PATH; Fortran layer not built", flush=True)
         return None
     os.makedirs(OUT, exist_ok=True)
-    flags = ["-O2", "-module-dir", OUT]
+    flags = ["-O2", "-fPIC", "-module-dir", OUT]
     objs = []
     for src in MODULES + ["adz_f_driver.F90"]:
         obj = os.path.join(OUT, src.replace(".F90", ".o"))
@@ -41,13 +44,15 @@ def build(verbose=False):
         subprocess.check_call(cmd)
         objs.append(obj)
     rocm = os.environ.get("ROCM_PATH", "/opt/rocm")
-    link = [fc, "-o", DRIVER + ".tmp"] + objs + ["-L" + PKG, "-ladz_hip", "-Wl,-rpath,$ORIGIN/..",
-                                                "-L" + os.path.join(rocm, "lib"), "-lamdhip64",
-                                                "-Wl,-rpath," + os.path.join(rocm, "lib")]
-    if verbose:
-        print(" ".join(link), flush=True)
-    subprocess.check_call(link)
-    os.replace(DRIVER + ".tmp", DRIVER)
+    libs = ["-L" + PKG, "-ladz_hip", "-Wl,-rpath,$ORIGIN/..", "-L" + os.path.join(rocm, "lib"), "-lamdhip64",
+            "-Wl,-rpath," + os.path.join(rocm, "lib")]
+    shared = [fc, "-shared", "-o", SHLIB + ".tmp"] + objs[:len(MODULES)] + libs
+    link = [fc, "-o", DRIVER + ".tmp"] + objs + libs
+    for cmd, out in ((shared, SHLIB), (link, DRIVER)):
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.check_call(cmd)
+        os.replace(out + ".tmp", out)
     return DRIVER
 
 

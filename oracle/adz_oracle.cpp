@@ -17,6 +17,9 @@ template <class TP>
 const TP* P3(const void* p, long i) { return (const TP*)p + 3 * i; }
 }  // namespace
 
+// threads of the OpenMP loops (the CPU baselines time all cores and one core)
+extern "C" void ora_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
+
 // ---------------------------- vertical descriptor -----------------------------------
 extern "C" void* ora_ztab_new(const double* levels, int nk, int ni, int nj, int offi, int offj) {
   return new Ztab(ztab_setup(levels, nk, ni, nj, offi, offj));

@@ -68,8 +68,14 @@ def lib():
         L.ora_ilmc.argtypes = [vp, vp, vp, vp, vp, vp]
         L.ora_ilmc_shell.restype = l
         L.ora_ilmc_shell.argtypes = [vp, i, i, i, i, vp]
+        L.ora_set_threads.argtypes = [i]
         _lib = L
     return _lib
+
+
+def set_threads(n):
+    """threads of the oracle's OpenMP loops"""
+    lib().ora_set_threads(int(n))
 
 
 def _p(a):

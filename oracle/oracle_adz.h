@@ -260,6 +260,7 @@ inline void prepare_winds(const Grid& g, const float* uu_ext, const float* vv_ex
   for (int k = 3; k <= nk; k++) wx[k] = g.vw1m[k];
   auto km2 = [](int k) { return std::max(1, k - 2); };
   const double ir = g.d.inv_rayt;
+#pragma omp parallel for schedule(static)
   for (int k = 1; k <= nk; k++)
     for (int j = g.lminy; j <= g.lmaxy; j++)
       for (int i = g.lminx; i <= g.lmaxx; i++) {
@@ -268,6 +269,7 @@ inline void prepare_winds(const Grid& g, const float* uu_ext, const float* vv_ex
         s->uvw_d[3 * q + 1] = (S)(ir * (double)vv_ext[q]);
       }
   const long nxy = (long)g.ni * g.nj;
+#pragma omp parallel for schedule(static)
   for (int k = 1; k <= nk; k++)
     for (int j = 1; j <= g.nj; j++)
       for (int i = 1; i <= g.ni; i++) {
@@ -279,17 +281,20 @@ inline void prepare_winds(const Grid& g, const float* uu_ext, const float* vv_ex
         S v2 = (S)vt0[e.idx(i, j, k)] + (S)vt0[e.idx(i, j - 1, k)];
         s->vv_arr[a] = (S)(((double)v1 * alpha1 + (double)v2 * alpha2) * ir);
       }
+#pragma omp parallel for schedule(static)
   for (int j = g.lminy; j <= g.lmaxy; j++)
     for (int i = g.lminx; i <= g.lmaxx; i++) {
       s->uvw_d[3 * e.idx(i, j, 1) + 2] = (S)(g.vw5 * (double)ww_ext[e.idx(i, j, 1)]);
       s->uvw_d[3 * e.idx(i, j, nk) + 2] = (S)(g.vw6 * (double)ww_ext[e.idx(i, j, nk - 1)]);
     }
+#pragma omp parallel for schedule(static)
   for (int j = 1; j <= g.nj; j++)
     for (int i = 1; i <= g.ni; i++) {
       long a = (i - 1) + (long)g.ni * (j - 1);
       s->ww_arr[a] = (S)(g.vw5 * (double)zdt0[e.idx(i, j, 1)]);
       s->ww_arr[a + nxy * (nk - 1)] = (S)(g.vw6 * (double)zdt0[e.idx(i, j, nk - 1)]);
     }
+#pragma omp parallel for schedule(static)
   for (int k = 2; k <= nk - 1; k++)
     for (int j = 1; j <= g.nj; j++)
       for (int i = 1; i <= g.ni; i++) {
@@ -298,6 +303,7 @@ inline void prepare_winds(const Grid& g, const float* uu_ext, const float* vv_ex
                    g.vw3m[k] * (double)zdt0[e.idx(i, j, k)] + g.vw4m[k] * (double)zdt0[e.idx(i, j, k + 1)];
         s->ww_arr[a] = (S)w;
       }
+#pragma omp parallel for schedule(static)
   for (int k = 2; k <= nk - 1; k++)
     for (int j = g.lminy; j <= g.lmaxy; j++)
       for (int i = g.lminx; i <= g.lmaxx; i++) {

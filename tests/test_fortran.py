@@ -20,7 +20,20 @@ def test_bind_c_type_sizes_match_c_abi():
     line = [ln for ln in out.stdout.splitlines() if ln.startswith("sizes")][0]
     sizes = [int(x) for x in line.split()[1:]]
     assert sizes == [C.sizeof(s) for s in (A.AdzGrid, A.AdzField, A.AdzItem, A.AdzTune, A.AdzBcTracer,
-                                           A.AdzBcDomain, A.AdzFluxTracer, A.AdzFluxDomain)]
+                                           A.AdzBcDomain, A.AdzFluxTracer, A.AdzFluxDomain,
+                                           A.AdzAirmassFields)]
+
+
+def test_fortran_layer_library_exports_its_entry_points():
+    """libadz_gpu.so (module adz_gpu and its C-callable entry points) loads on a host
+    without a GPU and exports every routine gem_amd.adz_fortran binds."""
+    from gem_amd import adz_fortran as AF
+
+    if not os.path.exists(AF.LIB_PATH):
+        pytest.skip("Fortran layer not built (no amdflang)")
+    F = AF.flib()
+    for name in AF._SIGS:
+        assert hasattr(F, name), name
 
 
 @pytest.mark.parametrize("what", ["ilmc", "qv"])

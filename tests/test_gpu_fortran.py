@@ -8,7 +8,8 @@ the LAM boundary flux, Adz_BC_LAM_flux = 1).  Both interpolation forms: tricubic
 bicubic-horizontal / quintic-vertical (F_QV_L = .true.); and the ZLF boundary treatment
 (Adz_BC_LAM_flux = 2: interpolation on Adz_pb, piloting values saved and restored, the
 fixer on the Adz_pb box); and ILMC (tracers with mono = 2, ILMC_LAM called by
-adz_post_tr(1) before the fixer)."""
+adz_post_tr(1) before the fixer); and the tricubic step captured by the Fortran program in
+a hipGraph on a stream it created and replayed (the replay's output is compared)."""
 import os
 import subprocess
 
@@ -23,7 +24,7 @@ DRIVER = os.path.join(ROOT, "gem_amd", "fortran", "adz_f_driver")
 MH = 2
 
 
-@pytest.mark.parametrize("form", ["tricubic", "quintic", "zlf", "ilmc"])
+@pytest.mark.parametrize("form", ["tricubic", "quintic", "zlf", "ilmc", "graph"])
 def test_fortran_step_matches_c_abi(cuda, tmp_path, form):
     if not os.path.exists(DRIVER):
         pytest.fail("gem_amd/fortran/adz_f_driver is not built")
@@ -66,7 +67,7 @@ def test_fortran_step_matches_c_abi(cuda, tmp_path, form):
         mass.tofile(f)
         mono.tofile(f)
     print("[gpu-case] running the Fortran driver", flush=True)
-    mode = {"tricubic": [], "quintic": ["qv"], "zlf": ["zlf"], "ilmc": []}[form]
+    mode = {"tricubic": [], "quintic": ["qv"], "zlf": ["zlf"], "ilmc": [], "graph": ["graph"]}[form]
     r = subprocess.run([DRIVER, str(fin), str(fout)] + mode, capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
 
