@@ -115,7 +115,7 @@ __device__ __forceinline__ void store_out(const Fields& F, int f, long o, TC v, 
   } else {
     if (POST) {
       TF ls = from_f32<TF>(l), mis = from_f32<TF>(mi), mas = from_f32<TF>(ma);
-      TF vs = from_f32<TF>(v);
+      TF vs = from_f32<TF>(snap_to_range(v, mi, ma));
       if (POST & ADZ_POST_CLIP) vs = from_f32<TF>(fmaxf(to_f32(mis), fminf(to_f32(mas), to_f32(vs))));
       d[o] = vs;
       if (POST & ADZ_POST_STORE) {
@@ -1046,6 +1046,9 @@ int adz_plan_create(const adz_item* items, int nitems, int fdtype, int math, int
   for (int q = 0; q < nitems; q++)
     if (items[q].pdtype != pdt) return fail(ADZ_EINVAL, "adz_plan_create: items must share the position dtype");
   if (!pick<true>(fdtype, pdt, math, post)) return fail(ADZ_EINVAL, "adz_plan_create: unsupported dtype/math");
+  if (tune && (tune->stage < 0 || tune->stage > 4)) return fail(ADZ_EINVAL, "adz_plan_create: stage must be 0..4");
+  if (tune && (tune->xcd_chunks < 0 || tune->xcd_chunks > 1))
+    return fail(ADZ_EINVAL, "adz_plan_create: xcd_chunks must be 0 or 1");
   adz_tune T{8, 0, 4, 1, 2, 0};
   if (tune) {
     if (tune->fields_per_pass > 0) T.fields_per_pass = std::min(64, tune->fields_per_pass);

@@ -118,6 +118,20 @@ template <class TF, class P>
 __device__ __forceinline__ float tricub_f32(P f1, const float* c, long ni, long o1, long o2, long o3) {
   return tricub_f32_a<TF>(PtrStencil<P>{f1, ni, o1, o2, o3}, c);
 }
+// The FP32 form's cubic value put on its stencil's inner range [mi, ma] when it lies
+// outside by at most 2^-20 of the range's larger magnitude (8 to 16 ulp).  FP32 weights do
+// not sum to exactly 1, so a constant neighbourhood comes out a few ulp off the constant,
+// outside [mi, ma] = [c, c], where the reference's FP64 arithmetic narrowed to FP32 gives c
+// itself; a shape-preserving pass (clipping, ILMC, the fixer's min/max reset) would then
+// treat every such cell as a violation.  Values farther out (genuine overshoots) and NaN
+// are left as they are.  The reference-arithmetic form is not touched.
+__device__ __forceinline__ float snap_to_range(float v, float mi, float ma) {
+  const float tol = fmaxf(fabsf(mi), fabsf(ma)) * 0x1p-20f;
+  if (v > ma && v - ma <= tol) return ma;
+  if (v < mi && mi - v <= tol) return mi;
+  return v;
+}
+
 template <class TF, class TC, class P>
 __device__ __forceinline__ void mono_extra(P f1, const TC* c, long ni, long o1, long o2, TC* lin, TC* mi, TC* ma) {
   mono_extra_a<TF, TC>(PtrStencil<P>{f1, ni, o1, o2, o1 + o2}, c, lin, mi, ma);

@@ -73,12 +73,14 @@ __global__ __launch_bounds__(256) void k_tricub_m_n(TF* __restrict__ d, TF* __re
         continue;
       }
       const TF* f1 = (const TF*)fs.f[f] + x.off;
-      d[o] = from_f32<TF>(tricub_f32<TF>(f1, c, ni, o1, o2, o3));
+      float v = tricub_f32<TF>(f1, c, ni, o1, o2, o3);
       if (KIND == K_MONO) {
         float a, b, e;
         mono_extra<TF, float>(f1, c, ni, o1, o2, &a, &b, &e);
+        v = snap_to_range(v, b, e);
         l[o] = from_f32<TF>(a); mi[o] = from_f32<TF>(b); ma[o] = from_f32<TF>(e);
       }
+      d[o] = from_f32<TF>(v);
     }
   }
 }

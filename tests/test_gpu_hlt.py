@@ -195,7 +195,9 @@ def _plan_case(T):
                                   dict(fields_per_pass=5, points_outer=1, blocks_per_cu=2, xcd_chunks=0, stage=1,
                                        level_inner=3),
                                   dict(fields_per_pass=3, points_outer=0, blocks_per_cu=3, xcd_chunks=1, stage=3,
-                                       level_inner=2)])
+                                       level_inner=2),
+                                  dict(fields_per_pass=2, points_outer=0, blocks_per_cu=4, xcd_chunks=1, stage=4,
+                                       level_inner=4)])
 def test_plan_matches_single_calls(T, tune):
     torch, L, A = T
     items, refs, keep = _plan_case(T)
@@ -355,4 +357,59 @@ def test_shared_columns_edge_positions(T, math):
             nan_seen = nan_seen or bool(torch.isnan(a[k]).any())
     assert nan_seen
     plan.free()
+    ctx.free()
+
+
+@pytest.mark.parametrize("entry", ["hlt", "plan", "mono_m_n"])
+def test_f32_form_reproduces_constants(T, entry):
+    """The FP32 form on a plume over a constant 1e-12 background (the c5 tracers' shape):
+    wherever a target's stencil range is a single value (min == max) the value is that
+    constant, so no cell of a constant neighbourhood lies outside [min, max]; elsewhere the
+    values stay within the FP32 form's error of the FP64 oracle."""
+    torch, L, A = T
+    from gem_amd import tricublin as TC
+
+    g, ctx, _ = _setup(T, CFG, L.ADZ_F32, "f32", nfields=1)
+    i = ctx.info
+    box = (i["i0"], i["in"], i["j0"], i["jn"], i["k0t"], CFG.nk)
+    shape = (CFG.nk, i["njt"], i["nit"])
+    rng = np.random.default_rng(12)
+    src64 = S.plume_tracer(shape, rng, "cpu", background=1e-12, nplumes=2).reshape(-1).numpy()
+    src = torch.from_numpy(src64.astype(np.float32)).cuda()
+    src_off = (1 - i["lminx"]) + i["nit"] * (1 - i["lminy"])
+    pt = ctx.array(A.ADZ_A_PT)
+    n = pt.numel() // 3
+    if entry == "mono_m_n":
+        outs = [torch.zeros(n, device="cuda") for _ in range(4)]
+        lv = TC.Ztab(g["zt"][1:CFG.nk + 1], i["nit"], i["njt"])
+        TC.Tricublin_mono_zyx_m_n(*outs, [src], pt, lv, n, 1, base_off=src_off, math=L.ADZ_MATH_F32)
+        torch.cuda.synchronize()
+        d, mi, ma = outs[0], outs[2], outs[3]
+    else:
+        n_model = (CFG.ni + 2 * MH) * (CFG.nj + 2 * MH) * CFG.nk
+        e = {k: torch.zeros(n_model, device="cuda") for k in ("dst", "lin", "min", "max")}
+        e["src"] = src
+        dl = (1 - MH, CFG.ni + MH, 1 - MH, CFG.nj + MH)
+        if entry == "hlt":
+            A.adz_tricub_hlt([e], pt, ctx.ztab(True), *box[:5], box[5], dl, post=A.ADZ_POST_STORE,
+                             math=L.ADZ_MATH_F32, src_off=src_off)
+        else:
+            dst_off = (1 - dl[0]) + (dl[1] - dl[0] + 1) * (1 - dl[2])
+            item = dict(geom=ctx.ztab(True), xyz=pt, box=box, dlayout=dl, fields=A.field_list([e], src_off, dst_off))
+            plan = A.Plan([item], L.ADZ_F32, math=L.ADZ_MATH_F32, post=A.ADZ_POST_STORE)
+            plan.execute()
+            torch.cuda.synchronize()
+            plan.free()
+        torch.cuda.synchronize()
+        sl = lambda a: a.reshape(CFG.nk, CFG.nj + 2 * MH, CFG.ni + 2 * MH)[
+            box[4] - 1:, box[2] - 1 + MH:box[3] + MH, box[0] - 1 + MH:box[1] + MH].reshape(-1)
+        d, mi, ma = sl(e["dst"]), sl(e["min"]), sl(e["max"])
+    const = mi == ma
+    assert int(const.sum()) > n // 4  # the background dominates
+    assert torch.equal(d[const], mi[const])
+    assert int(((d < mi) | (d > ma))[const].sum()) == 0
+    zt = O.Ztab(g["zt"][1:CFG.nk + 1], i["nit"], i["njt"], 0, 0)
+    exp = O.zyx1_m_n(zt, [src64], src_off, pt.cpu().numpy(), O.ORD_AVX2)[0]
+    got = d.double().cpu().numpy()
+    assert np.abs(got - exp).max() <= 5e-6 * np.abs(exp).max()
     ctx.free()
