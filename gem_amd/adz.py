@@ -342,7 +342,7 @@ class AdzBcTracer(C.Structure):
 
 class AdzBcDomain(C.Structure):
     _fields_ = [(n, C.c_int) for n in ("minx", "maxx", "miny", "maxy", "nk", "i0", "in_", "j0", "jn", "k0",
-                                       "min_max")] + [("pexp_list", C.c_float * 9)]
+                                       "min_max")] + [("pexp_list", C.c_float * 9), ("autobar", C.c_int)]
 
 
 def _check_types(dtype, **arrays):
@@ -364,9 +364,10 @@ class BermejoConde:
     """Bermejo_Conde(F_bc, F_ntr_bc, F_i0, F_in, F_j0, F_jn, F_k0) on device arrays.
 
     tracers: list of dicts with tensors p, m, lin, min, max (model layout, element (1,1,1)
-    at offset `off`) and the tracer's `mass` code (1 or 100+10*weight+pexp_n)."""
+    at offset `off`) and the tracer's `mass` code (1 or 100+10*weight+pexp_n).  autobar:
+    Schm_autobar_L (the masses are level-1 sums with geomh_area_mask_8 given as `mask`)."""
 
-    def __init__(self, tracers, layout, scope, off, min_max=True, pexp_list=(1.0,) * 9):
+    def __init__(self, tracers, layout, scope, off, min_max=True, pexp_list=(1.0,) * 9, autobar=False):
         minx, maxx, miny, maxy, nk = layout
         i0, in_, j0, jn, k0 = scope
         arr = (AdzBcTracer * len(tracers))()
@@ -379,7 +380,7 @@ class BermejoConde:
             a.legacy, a.weight, a.pexp_n = legacy, weight, pexp_n
             tdt = _DT[t["m"].dtype]
         dom = AdzBcDomain(minx, maxx, miny, maxy, nk, i0, in_, j0, jn, k0, 1 if min_max else 0,
-                          (C.c_float * 9)(*pexp_list))
+                          (C.c_float * 9)(*pexp_list), 1 if autobar else 0)
         h = C.c_void_p()
         L.check(L.lib().adz_bc_create(arr, len(tracers), C.byref(dom), tdt, C.byref(h)), "adz_bc_create")
         self.h, self.off, self.ntr = h, off, len(tracers)
@@ -480,7 +481,7 @@ class AdzFluxTracer(C.Structure):
 
 class AdzFluxDomain(C.Structure):
     _fields_ = [(n, C.c_int) for n in ("minx", "maxx", "miny", "maxy", "nk", "ni", "nj", "i0b", "inb", "j0b", "jnb",
-                                       "ic0", "ic1", "jc0", "jc1", "kc0")]
+                                       "ic0", "ic1", "jc0", "jc1", "kc0", "autobar")]
 
 
 class BcLamFlux:
@@ -491,9 +492,10 @@ class BcLamFlux:
     (Adz_flux%fo / %fi, model layout, element (1,1,1) at offset `off`).  layout =
     (minx, maxx, miny, maxy, nk); tile = (ni, nj); bbox = (i0b, inb, j0b, jnb) of the
     Adz_pb targets; core = (i0, in, j0, jn, k0) of the fixer's scope.  geom: the thermo
-    descriptor (AdzContext.ztab(True)); pos_dtype: storage of the positions."""
+    descriptor (AdzContext.ztab(True)); pos_dtype: storage of the positions; autobar:
+    Schm_autobar_L (level-1 masses with geomh_area_mask_8 given as `mask`)."""
 
-    def __init__(self, tracers, layout, tile, bbox, core, off, geom, pos_dtype=L.ADZ_F32):
+    def __init__(self, tracers, layout, tile, bbox, core, off, geom, pos_dtype=L.ADZ_F32, autobar=False):
         minx, maxx, miny, maxy, nk = layout
         arr = (AdzFluxTracer * len(tracers))()
         tdt = None
@@ -502,7 +504,7 @@ class BcLamFlux:
             arr[n].fo = _ptr_at(t.get("fo"), off)
             arr[n].fi = _ptr_at(t.get("fi"), off)
             tdt = _DT[t["src"].dtype]
-        dom = AdzFluxDomain(minx, maxx, miny, maxy, nk, *tile, *bbox, *core)
+        dom = AdzFluxDomain(minx, maxx, miny, maxy, nk, *tile, *bbox, *core, 1 if autobar else 0)
         h = C.c_void_p()
         L.check(L.lib().adz_flux_create(arr, len(tracers), C.byref(dom), geom, tdt, pos_dtype, C.byref(h)),
                 "adz_flux_create")

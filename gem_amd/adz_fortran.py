@@ -34,6 +34,7 @@ _SIGS = {
     "adz_f_set_bc_fields": (None, ["vp"] * 4),
     "adz_f_set_airmass": (None, ["vp"] * 5 + ["i", "i"]),
     "adz_f_set_options": (None, ["i"] * 4),
+    "adz_f_set_area_mask": (None, ["vp"]),
     "adz_f_set_tune": (None, ["i", "vp"]),
     "adz_f_inittraj": (None, []),
     "adz_f_main": (None, []),
@@ -137,6 +138,11 @@ class FortranStep:
         flib().adz_f_set_airmass(_p(pm_p, off), _p(pm_m, off), _p(pt_m, off), _p(p0_m, area_off),
                                  _p(area, area_off), int(hu), 1 if dry else 0)
         self._keep.append((pm_p, pm_m, pt_m, p0_m, area))
+
+    def set_area_mask(self, area_mask, off):
+        """geomh_area_mask_8 (FP64, 2-D), the weight of the masses under Schm_autobar_L"""
+        flib().adz_f_set_area_mask(_p(area_mask, off))
+        self._keep.append(area_mask)
 
     def set_options(self, bc_min_max=True, ilmc_sweep_max=2, ilmc_min_max=True, autobar=False):
         flib().adz_f_set_options(int(bc_min_max), int(ilmc_sweep_max), int(ilmc_min_max), int(autobar))

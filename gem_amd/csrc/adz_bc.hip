@@ -18,6 +18,7 @@
 // tracer.  Scalars stay on the device, so a timestep can be captured in a hipGraph.
 #include <algorithm>
 #include <cstring>
+#include <string>
 #include <vector>
 
 #include "adz_common.h"
@@ -50,6 +51,7 @@ struct BcArgs {
   int i0, j0, k0, nx, ny;    // scope
   long npts;
   int min_max;
+  int autobar;               // Schm_autobar_L: masses over level 1 of x * mask (area mask)
   int nblk;                  // blocks per tracer
   int nrows;                 // scope rows (j fastest, then k)
   const void* airm1;
@@ -88,6 +90,14 @@ __device__ __forceinline__ T bc_weight(T m, T lin, double pkps, const BcTr& t, d
     w = (T)v;
   }
   return w;
+}
+
+// one element's contribution to a mass: tracer * air mass * geomh_mask_8 (mass_tr_X.F90:86-93),
+// or, with Schm_autobar_L, tracer * geomh_area_mask_8 on level 1 only (:73-79)
+template <class T>
+__device__ __forceinline__ double bc_term(int autobar, int k, T x, T air, double mask) {
+  if (autobar) return k == 1 ? (double)x * mask : 0.0;
+  return mass_term<T>(x, air, mask);
 }
 
 // |w| and the sign of m - lin for the fused first pass (w = sign * |w| when the signs match)
@@ -174,14 +184,14 @@ __global__ __launch_bounds__(kBcBlock) void k_bc_pass(BcArgs A) {
         ADZ_GLOBAL T* mp = mg + qr + ii;
         T m = mv[u];
         if (PASS == P_PM) {
-          dd_add(&s0, mass_term<T>(nv[u], bv[u], mk[u]));
-          dd_add(&s1, mass_term<T>(m, av[u], mk[u]));
+          dd_add(&s0, bc_term<T>(A.autobar, k, nv[u], bv[u], mk[u]));
+          dd_add(&s1, bc_term<T>(A.autobar, k, m, av[u], mk[u]));
           bool pos;
-          const double wt = mass_term<T>(bc_weight_abs<T>(m, lv[u], pkv[u], t, &pos), av[u], mk[u]);
+          const double wt = bc_term<T>(A.autobar, k, bc_weight_abs<T>(m, lv[u], pkv[u], t, &pos), av[u], mk[u]);
           dd_add(pos ? &s2 : &s3, wt);
         } else if (PASS == P_W) {
           T w = bc_weight<T>(m, lv[u], pkv[u], t, sd);
-          dd_add(&s0, mass_term<T>(w, av[u], mk[u]));
+          dd_add(&s0, bc_term<T>(A.autobar, k, w, av[u], mk[u]));
         } else if (PASS == P_CORR) {
           if (!cyc) {
             T w = bc_weight<T>(m, lv[u], pkv[u], t, sd);
@@ -194,13 +204,13 @@ __global__ __launch_bounds__(kBcBlock) void k_bc_pass(BcArgs A) {
             m = bc;
             *mp = m;
           }
-          dd_add(&s0, mass_term<T>(m, av[u], mk[u]));
+          dd_add(&s0, bc_term<T>(A.autobar, k, m, av[u], mk[u]));
         } else {  // P_PROP
           if (!cyc) {
             m = (T)(prop * (double)m);
             *mp = m;
           }
-          dd_add(&s0, mass_term<T>(m, av[u], mk[u]));
+          dd_add(&s0, bc_term<T>(A.autobar, k, m, av[u], mk[u]));
         }
       }
     }
@@ -297,23 +307,27 @@ struct adz_bc {
   void* mem;
 };
 
-extern "C" {
+namespace {
 
-int adz_bc_create(const adz_bc_tracer* tr, int ntr, const adz_bc_domain* dom, int tdtype, adz_bc** out) {
-  if (!tr || ntr <= 0 || !dom || !out) return fail(ADZ_EINVAL, "adz_bc_create: bad argument");
-  if (tdtype != ADZ_F32 && tdtype != ADZ_F64) return fail(ADZ_EINVAL, "adz_bc_create: tdtype must be F32 or F64");
+constexpr long kBcMaxBlocks = 2048;  // row blocks per tracer (partials kept per block)
+
+// Validate the tracers and scope of a fixer call and fill the host tracer table h[0..ntr)
+// and the launch arguments (without device addresses); `who` names the entry point.
+int bc_setup(const adz_bc_tracer* tr, int ntr, const adz_bc_domain* dom, const char* who, BcTr* h, BcArgs* out,
+             int* has_w3) {
   const adz_bc_domain& d = *dom;
   if (d.in < d.i0 || d.jn < d.j0 || d.nk < d.k0 || d.i0 < d.minx || d.in > d.maxx || d.j0 < d.miny ||
       d.jn > d.maxy || d.k0 < 1)
-    return fail(ADZ_EINVAL, "adz_bc_create: scope outside the arrays");
-  std::vector<BcTr> h(ntr);
+    return fail(ADZ_EINVAL, std::string(who) + ": scope outside the arrays");
+  *has_w3 = 0;
   for (int n = 0; n < ntr; n++) {
     const adz_bc_tracer& t = tr[n];
-    if (!t.p || !t.m || !t.lin || !t.mn || !t.mx) return fail(ADZ_EINVAL, "adz_bc_create: null tracer array");
+    if (!t.p || !t.m || !t.lin || !t.mn || !t.mx) return fail(ADZ_EINVAL, std::string(who) + ": null tracer array");
     if (t.weight < 1 || t.weight > 3) return fail(ADZ_EINVAL, "BERMEJO-CONDE: BC WEIGHT not valid");
     if (!t.legacy && (t.pexp_n < 1 || t.pexp_n > 9)) return fail(ADZ_EINVAL, "BERMEJO-CONDE: BC PEXP not valid");
     h[n] = BcTr{t.p, t.m, t.lin, t.mn, t.mx, t.legacy, t.weight,
                 t.legacy ? 1.0 : (double)d.pexp_list[t.pexp_n - 1]};
+    *has_w3 |= (t.weight == 3);
   }
   BcArgs a;
   std::memset(&a, 0, sizeof(a));
@@ -322,14 +336,32 @@ int adz_bc_create(const adz_bc_tracer* tr, int ntr, const adz_bc_domain* dom, in
   a.i0 = d.i0; a.j0 = d.j0; a.k0 = d.k0; a.nx = d.in - d.i0 + 1; a.ny = d.jn - d.j0 + 1;
   a.npts = (long)a.nx * a.ny * (d.nk - d.k0 + 1);
   a.min_max = d.min_max;
+  a.autobar = d.autobar != 0;
+  if (a.autobar && d.k0 != 1) return fail(ADZ_EINVAL, std::string(who) + ": Schm_autobar_L needs the scope from level 1");
   a.nrows = a.ny * (d.nk - d.k0 + 1);
   long per = (long)kBcBlock * 16;
   long nblk = (a.npts + per - 1) / per;
-  if (nblk > 2048) nblk = 2048;
+  if (nblk > kBcMaxBlocks) nblk = kBcMaxBlocks;
   if (nblk > a.nrows) nblk = a.nrows;
   if (nblk < 1) nblk = 1;
   nblk = std::max(8L, nblk / 8 * 8);  // ntr*nblk a multiple of 8 (one eighth per XCD)
   a.nblk = (int)nblk;
+  *out = a;
+  return ADZ_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int adz_bc_create(const adz_bc_tracer* tr, int ntr, const adz_bc_domain* dom, int tdtype, adz_bc** out) {
+  if (!tr || ntr <= 0 || !dom || !out) return fail(ADZ_EINVAL, "adz_bc_create: bad argument");
+  if (tdtype != ADZ_F32 && tdtype != ADZ_F64) return fail(ADZ_EINVAL, "adz_bc_create: tdtype must be F32 or F64");
+  std::vector<BcTr> h(ntr);
+  BcArgs a;
+  int has_w3 = 0;
+  if (int rc = bc_setup(tr, ntr, dom, "adz_bc_create", h.data(), &a, &has_w3)) return rc;
+  const long nblk = a.nblk;
   size_t tb = sizeof(BcTr) * ntr, pb = sizeof(double) * 8 * ntr * nblk, sb = sizeof(double) * kSc * ntr;
   void* mem = nullptr;
   if (hipMalloc(&mem, tb + pb + sb) != hipSuccess) return fail(ADZ_ENOMEM, "adz_bc_create: hipMalloc");
@@ -343,8 +375,7 @@ int adz_bc_create(const adz_bc_tracer* tr, int ntr, const adz_bc_domain* dom, in
   a.sc = (double*)((char*)mem + tb + pb);
   adz_bc* b = new adz_bc;
   b->tdtype = tdtype; b->a = a; b->mem = mem;
-  b->has_w3 = 0;
-  for (int n = 0; n < ntr; n++) b->has_w3 |= (tr[n].weight == 3);
+  b->has_w3 = has_w3;
   *out = b;
   return ADZ_OK;
 }
@@ -353,13 +384,13 @@ int adz_bc_create(const adz_bc_tracer* tr, int ntr, const adz_bc_domain* dom, in
 
 namespace {
 
-int bc_apply(adz_bc* bc, const void* airm1, const void* airm0, const void* pkps, const double* mask,
-             const double* flux, double* deficit, double* diag, adz_stream stream) {
-  if (!bc || !airm1 || !airm0 || !mask || !deficit) return fail(ADZ_EINVAL, "adz_bermejo_conde: null argument");
-  BcArgs a = bc->a;
+// The fixer's passes on the stream: a.tr, a.part and a.sc address device memory.
+int bc_launch(BcArgs a, int tdtype, int has_w3, const void* airm1, const void* airm0, const void* pkps,
+              const double* mask, const double* flux, double* deficit, double* diag, adz_stream stream) {
+  if (!airm1 || !airm0 || !mask || !deficit) return fail(ADZ_EINVAL, "adz_bermejo_conde: null argument");
   a.airm1 = airm1; a.airm0 = airm0; a.pkps = pkps; a.mask = mask; a.deficit = deficit; a.diag = diag;
   a.flux = flux;
-  if (bc->has_w3 && !pkps) return fail(ADZ_EINVAL, "adz_bermejo_conde: weight 3 needs pkps");
+  if (has_w3 && !pkps) return fail(ADZ_EINVAL, "adz_bermejo_conde: weight 3 needs pkps");
   hipStream_t st = (hipStream_t)stream;
   dim3 g((unsigned)((long)((a.ntr + kBcWaves - 1) / kBcWaves) * a.nblk)), b(kBcBlock);
   dim3 gf((unsigned)a.ntr), bf(kFinBlock);
@@ -372,7 +403,7 @@ int bc_apply(adz_bc* bc, const void* airm1, const void* airm0, const void* pkps,
     hipLaunchKernelGGL((k_bc_pass<T, P_PROP>), g, b, 0, st, a);               \
     hipLaunchKernelGGL((k_bc_fin<P_PROP>), gf, bf, 0, st, a);                 \
   }
-  if (bc->tdtype == ADZ_F32) {
+  if (tdtype == ADZ_F32) {
     ADZ_BC_PASSES(float)
   } else {
     ADZ_BC_PASSES(double)
@@ -380,6 +411,12 @@ int bc_apply(adz_bc* bc, const void* airm1, const void* airm0, const void* pkps,
 #undef ADZ_BC_PASSES
   ADZ_HIP_CHECK(hipGetLastError());
   return ADZ_OK;
+}
+
+int bc_apply(adz_bc* bc, const void* airm1, const void* airm0, const void* pkps, const double* mask,
+             const double* flux, double* deficit, double* diag, adz_stream stream) {
+  if (!bc) return fail(ADZ_EINVAL, "adz_bermejo_conde: null argument");
+  return bc_launch(bc->a, bc->tdtype, bc->has_w3, airm1, airm0, pkps, mask, flux, deficit, diag, stream);
 }
 
 }  // namespace

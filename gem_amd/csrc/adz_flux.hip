@@ -52,6 +52,7 @@ struct FluxArgs {
   int nxa, nya;                  // array extents (row and plane pitch)
   long npts;
   int ntr, nslices, store;
+  int autobar;                   // Schm_autobar_L: masses over level 1 of FLUX * mask (area mask)
   const FluxTr* tr;
   unsigned char* cls;            // per target: bit0 active, bit1 in the core, bit2 bad position
   int* cnt;                      // per mark block: count, then exclusive offset within its chunk
@@ -242,7 +243,8 @@ __global__ __launch_bounds__(kFxBlock) void k_flux_eval(FluxArgs A) {
         v = tricub_ref_a<TF>(st, c);
       }
       const TF out = (TF)v;
-      const double term = mass_term<TF>(out, airv, mk);
+      // mass_tr_PM.F90:105-125: FLUX * air mass * mask, or on level 1 FLUX * area mask
+      const double term = A.autobar ? (k == 1 ? (double)out * mk : 0.0) : mass_term<TF>(out, airv, mk);
       const int a = 4 * g + (core ? 2 : 0);
       DD d{acc[a][threadIdx.x], acc[a + 1][threadIdx.x]};
       dd_add(&d, term);
@@ -354,6 +356,7 @@ int adz_flux_create(const adz_flux_tracer* tr, int ntr, const adz_flux_domain* d
   if (a.npts >= (1L << 30)) return fail(ADZ_EINVAL, "adz_flux_create: more than 2^30 targets");
   a.ntr = ntr;
   a.store = store;
+  a.autobar = d.autobar != 0;
   const int ngroups = (ntr + kFxGroup - 1) / kFxGroup;
   a.nslices = std::max(16, std::min(1024, 2048 / ngroups));
   const int nmark = (int)((a.npts + kFxBlock - 1) / kFxBlock);

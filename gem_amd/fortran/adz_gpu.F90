@@ -126,6 +126,8 @@ module adz_gpu
    logical :: Dynamics_hydro_L = .true., Dynamics_hauteur_L = .false.
    ! air mass at P and M, pr_k/pr_s and geomh_mask_8 read by Bermejo_Conde
    type(c_ptr) :: airm1 = c_null_ptr, airm0 = c_null_ptr, pkps = c_null_ptr, geomh_mask_8 = c_null_ptr
+   ! with Schm_autobar_L the masses are level-1 sums weighted by geomh_area_mask_8 (geomh)
+   type(c_ptr) :: geomh_area_mask_8 = c_null_ptr
    ! inputs of set_post_tr_hlt (gmm_pw, geomh, dyn_fisl_options, tdpack)
    type(c_ptr) :: pw_pm_plus = c_null_ptr, pw_pm_moins = c_null_ptr, pw_pt_moins = c_null_ptr, &
                   pw_p0_moins = c_null_ptr, geomh_area_8 = c_null_ptr
@@ -637,6 +639,20 @@ contains
    end subroutine adz_tracers
 
    ! ---------------------------------------------------------------- LAM boundary flux
+   ! the weight of the mass sums: geomh_mask_8, or geomh_area_mask_8 under Schm_autobar_L
+   ! (mass_tr_PM.F90:79-85, mass_tr_X.F90:73-79)
+   function mass_mask() result(p)
+      type(c_ptr) :: p
+      p = geomh_mask_8
+      if (Schm_autobar_L) then
+         if (.not. c_associated(geomh_area_mask_8)) then
+            write (error_unit, *) 'Schm_autobar_L: geomh_area_mask_8 must be set before the tracers'
+            error stop 1
+         end if
+         p = geomh_area_mask_8
+      end if
+   end function mass_mask
+
    logical function uses_bc(m)
       type(meta_tracers), intent(in) :: m
       uses_bc = m%mass == 1 .or. (m%mass >= 111 .and. m%mass <= 139)
@@ -699,12 +715,13 @@ contains
          Adz_flux_3CWP = flux_mem
       end if
       dom = adz_flux_domain(l_minx, l_maxx, l_miny, l_maxy, l_nk, l_ni, l_nj, Adz_i0b, Adz_inb, Adz_j0b, Adz_jnb, &
-                            1 + pil_w, l_ni - pil_e, 1 + pil_s, l_nj - pil_n, Adz_k0t)
+                            1 + pil_w, l_ni - pil_e, 1 + pil_s, l_nj - pil_n, Adz_k0t, merge(1, 0, Schm_autobar_L))
       call key_reset()
       call key_addi(nf); call key_addi(Adz_field_dtype); call key_addi(Adz_pos_dtype); call key_addp(Adz_cpntr_t)
       call key_addi(l_minx); call key_addi(l_maxx); call key_addi(l_miny); call key_addi(l_maxy); call key_addi(l_nk)
       call key_addi(Adz_i0b); call key_addi(Adz_inb); call key_addi(Adz_j0b); call key_addi(Adz_jnb)
       call key_addi(pil_w); call key_addi(pil_e); call key_addi(pil_s); call key_addi(pil_n); call key_addi(Adz_k0t)
+      call key_addi(merge(1, 0, Schm_autobar_L))
       do n = 1, nf
          call key_addp(tr(n)%src)
       end do
@@ -714,7 +731,7 @@ contains
                                         int(Adz_pos_dtype, c_int), h), 'adz_flux_create')
          call cache_put(K_FLUX, h)
       end if
-      call adz_check(c_adz_bc_lam_flux(h, Adz_pb, airm0, geomh_mask_8, Adz_flux, Adz_stream), 'adz_bc_lam_flux')
+      call adz_check(c_adz_bc_lam_flux(h, Adz_pb, airm0, mass_mask(), Adz_flux, Adz_stream), 'adz_bc_lam_flux')
       deallocate (tr)
    end subroutine adz_BC_LAM_Aranami_hlt
 
@@ -972,7 +989,7 @@ contains
                                merge(1, 0, F_bc(n)%LEGACY_L), F_bc(n)%weight, F_bc(n)%pexp_n)
       end do
       dom = adz_bc_domain(l_minx, l_maxx, l_miny, l_maxy, l_nk, F_i0, F_in, F_j0, F_jn, F_k0, &
-                          merge(1, 0, Adz_BC_min_max_L), Adz_BC_pexp_list)
+                          merge(1, 0, Adz_BC_min_max_L), Adz_BC_pexp_list, merge(1, 0, Schm_autobar_L))
       ! the device deficits: one per tracer of F_bc, carried from call to call
       if (F_ntr_bc > deficit_cap .or. .not. c_associated(Adz_bc_deficit)) then
          if (c_associated(Adz_bc_deficit)) then
@@ -999,7 +1016,7 @@ contains
       call key_addi(F_ntr_bc); call key_addi(Adz_field_dtype)
       call key_addi(l_minx); call key_addi(l_maxx); call key_addi(l_miny); call key_addi(l_maxy); call key_addi(l_nk)
       call key_addi(F_i0); call key_addi(F_in); call key_addi(F_j0); call key_addi(F_jn); call key_addi(F_k0)
-      call key_addi(merge(1, 0, Adz_BC_min_max_L))
+      call key_addi(merge(1, 0, Adz_BC_min_max_L)); call key_addi(merge(1, 0, Schm_autobar_L))
       do n = 1, 9
          call key_add(int(transfer(Adz_BC_pexp_list(n), 0_c_int32_t), c_int64_t))
       end do
@@ -1013,10 +1030,10 @@ contains
          call cache_put(K_BC, h)
       end if
       if (lam_flux_L) then  ! Bermejo_Conde.F90:70-72,92: the Aranami flux masses of each tracer
-         call adz_check(adz_bermejo_conde_lam(h, airm1, airm0, pkps, geomh_mask_8, flux_mem, Adz_bc_deficit, &
+         call adz_check(adz_bermejo_conde_lam(h, airm1, airm0, pkps, mass_mask(), flux_mem, Adz_bc_deficit, &
                                               c_null_ptr, Adz_stream), 'adz_bermejo_conde_lam')
       else
-         call adz_check(adz_bermejo_conde(h, airm1, airm0, pkps, geomh_mask_8, Adz_bc_deficit, c_null_ptr, &
+         call adz_check(adz_bermejo_conde(h, airm1, airm0, pkps, mass_mask(), Adz_bc_deficit, c_null_ptr, &
                                           Adz_stream), 'adz_bermejo_conde')
       end if
       deallocate (tr)

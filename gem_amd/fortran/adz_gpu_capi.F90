@@ -130,6 +130,12 @@ contains
       Schm_dry_mixing_ratio_L = dry /= 0
    end subroutine adz_f_set_airmass
 
+   ! geomh_area_mask_8 (element (1,1)): the weight of the masses under Schm_autobar_L
+   subroutine adz_f_set_area_mask(p) bind(C, name="adz_f_set_area_mask")
+      type(c_ptr), value :: p
+      geomh_area_mask_8 = p
+   end subroutine adz_f_set_area_mask
+
    ! options: Adz_BC_min_max_L, Adz_ILMC_sweep_max, Adz_ILMC_min_max_L, Schm_autobar_L
    subroutine adz_f_set_options(bc_min_max, ilmc_sweep_max, ilmc_min_max, autobar) bind(C, name="adz_f_set_options")
       integer(c_int), value :: bc_min_max, ilmc_sweep_max, ilmc_min_max, autobar

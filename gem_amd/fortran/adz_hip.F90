@@ -58,6 +58,7 @@ module adz_hip
       integer(c_int) :: i0, in, j0, jn, k0
       integer(c_int) :: min_max
       real(c_float)  :: pexp_list(9)
+      integer(c_int) :: autobar = 0      ! Schm_autobar_L: level-1 masses, mask = geomh_area_mask_8
    end type adz_bc_domain
 
    type, bind(C) :: adz_flux_tracer
@@ -86,6 +87,7 @@ module adz_hip
       integer(c_int) :: ni, nj
       integer(c_int) :: i0b, inb, j0b, jnb
       integer(c_int) :: ic0, ic1, jc0, jc1, kc0
+      integer(c_int) :: autobar = 0      ! Schm_autobar_L
    end type adz_flux_domain
 
    type, bind(C) :: adz_airmass_fields

@@ -237,7 +237,10 @@ typedef struct adz_tune {
   int stage;           /* 1: stage each tile's source box in LDS; 2: per-lane loads;
                           3 (FP32 sources): one producer wave per block streams the
                           boxes of several fields ahead into LDS (DMA), four consumer
-                          waves evaluate from LDS                                       */
+                          waves evaluate from LDS; 4 (FP32 sources): each lane loads one
+                          stencil column, reduces it for its own and its three left
+                          neighbours' targets, partials shared by lane shifts (61-wide
+                          tiles)                                                        */
   int level_inner;     /* tile order. 0: i, j, then k; 1: consecutive tiles walk the
                           levels of one tile column; S >= 2: super-columns of S tile
                           rows walked level by level (source planes and rows reused
@@ -293,6 +296,10 @@ typedef struct adz_bc_domain {
   int i0, in, j0, jn, k0;         /* scope of the sums and corrections (core, k0t:nk)  */
   int min_max;                    /* Adz_BC_min_max_L                                  */
   float pexp_list[9];             /* Adz_BC_pexp_list                                  */
+  int autobar;                    /* Schm_autobar_L: every mass is the sum over level 1
+                                     of tracer * mask with `mask` = geomh_area_mask_8
+                                     (mass_tr_PM.F90:79-85, mass_tr_X.F90:73-79); the
+                                     air masses are read but not used; needs k0 = 1   */
 } adz_bc_domain;
 
 typedef struct adz_bc adz_bc;
@@ -338,6 +345,9 @@ typedef struct adz_flux_domain {
   int ni, nj;                     /* the tile l_ni x l_nj                                       */
   int i0b, inb, j0b, jnb;         /* targets Adz_i0b:Adz_inb x Adz_j0b:Adz_jnb x 1:nk (Adz_pb)  */
   int ic0, ic1, jc0, jc1, kc0;    /* core 1+pil_w:l_ni-pil_e x 1+pil_s:l_nj-pil_n x Adz_k0t:nk  */
+  int autobar;                    /* Schm_autobar_L: the flux masses are sums over level 1 of
+                                     FLUX * mask with `mask` = geomh_area_mask_8
+                                     (mass_tr_PM.F90:105-112); the air mass is read, unused     */
 } adz_flux_domain;
 
 typedef struct adz_flux adz_flux;

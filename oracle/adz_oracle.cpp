@@ -214,7 +214,7 @@ static void bc_run(const int* dom, int ntr, const void* const* p, void* const* m
                    const int* pexp_n, double* deficit, const void* airm1, const void* airm0, const void* pkps,
                    const double* mask, const float* pexp_list, double* diag, const double* flux) {
   BCDomain D{dom[0], dom[1], dom[2], dom[3], dom[4], dom[5], dom[6], dom[7], dom[8], dom[9], mask, dom[10],
-             pexp_list, dom[11]};
+             pexp_list, dom[11], dom[12]};
   std::vector<BCTracer<T>> tr(ntr);
   for (int n = 0; n < ntr; n++)
     tr[n] = BCTracer<T>{(const T*)p[n], (T*)m[n], (const T*)lin[n], (const T*)mn[n], (const T*)mx[n],
@@ -235,8 +235,10 @@ extern "C" void ora_bermejo_conde(int tdt, const int* dom, int ntr, const void* 
     bc_run<double>(dom, ntr, p, m, lin, mn, mx, legacy, weight, pexp_n, deficit, airm1, airm0, pkps, mask, pexp_list,
                    diag, flux);
 }
+// dom: minx maxx miny maxy nk i0 in j0 jn k0 min_max sum_mode autobar
 extern "C" double ora_mass(int tdt, const int* dom, const void* x, const void* air, const double* mask) {
-  BCDomain D{dom[0], dom[1], dom[2], dom[3], dom[4], dom[5], dom[6], dom[7], dom[8], dom[9], mask, 0, nullptr, dom[11]};
+  BCDomain D{dom[0], dom[1], dom[2], dom[3], dom[4], dom[5], dom[6], dom[7], dom[8], dom[9], mask, 0, nullptr, dom[11],
+             dom[12]};
   if (tdt == 1) return mass_of(D, (const float*)x, (const float*)air);
   return mass_of(D, (const double*)x, (const double*)air);
 }

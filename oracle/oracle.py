@@ -329,11 +329,12 @@ class Grid:
 
 
 def bermejo_conde(tracers, layout, scope, airm1, airm0, mask, deficit, pkps=None, min_max=True,
-                  pexp_list=(1.0,) * 9, sum_mode=1, flux=None):
+                  pexp_list=(1.0,) * 9, sum_mode=1, flux=None, autobar=False):
     """Bermejo_Conde restatement on host arrays (array-start layout).  tracers: list of
     dicts with p, m (modified in place), lin, min, max and `mass` code.  sum_mode 1 =
     exactly rounded sums, 0 = the reference's sequential order.  flux: (mass_fo, mass_fi)
-    per tracer of the LAM Aranami flux (None: zero fluxes).  Returns (deficit, diag)."""
+    per tracer of the LAM Aranami flux (None: zero fluxes).  autobar: Schm_autobar_L (the
+    masses are level-1 sums and `mask` is geomh_area_mask_8).  Returns (deficit, diag)."""
     minx, maxx, miny, maxy, nk = layout
     i0, in_, j0, jn, k0 = scope
     ntr = len(tracers)
@@ -356,7 +357,8 @@ def bermejo_conde(tracers, layout, scope, airm1, airm0, mask, deficit, pkps=None
     legacy = np.array([c[0] for c in cs], np.int32)
     weight = np.array([c[1] for c in cs], np.int32)
     pexp_n = np.array([c[2] for c in cs], np.int32)
-    dom = np.array([minx, maxx, miny, maxy, nk, i0, in_, j0, jn, k0, 1 if min_max else 0, sum_mode], np.int32)
+    dom = np.array([minx, maxx, miny, maxy, nk, i0, in_, j0, jn, k0, 1 if min_max else 0, sum_mode,
+                    1 if autobar else 0], np.int32)
     de = np.ascontiguousarray(deficit, dtype=np.float64).copy()
     diag = np.zeros(6 * ntr)
     pk = pkps if pkps is not None else np.zeros_like(airm0)
@@ -367,10 +369,12 @@ def bermejo_conde(tracers, layout, scope, airm1, airm0, mask, deficit, pkps=None
     return de, diag.reshape(ntr, 6)
 
 
-def mass(x, air, mask, layout, scope, sum_mode=1):
+def mass(x, air, mask, layout, scope, sum_mode=1, autobar=False):
+    """mass_tr: sum of x * air * mask over the scope (autobar: level 1 of x * mask, with
+    mask = geomh_area_mask_8)"""
     minx, maxx, miny, maxy, nk = layout
     i0, in_, j0, jn, k0 = scope
-    dom = np.array([minx, maxx, miny, maxy, nk, i0, in_, j0, jn, k0, 0, sum_mode], np.int32)
+    dom = np.array([minx, maxx, miny, maxy, nk, i0, in_, j0, jn, k0, 0, sum_mode, 1 if autobar else 0], np.int32)
     return lib().ora_mass(1 if x.dtype == np.float32 else 0, _p(dom), _p(x), _p(air), _p(mask))
 
 
@@ -447,15 +451,16 @@ def bc_lam_aranami(tracers, layout, tile, bbox, core, zt, pb, order=ORD_AVX2, za
     return fo, fi
 
 
-def flux_masses(fo, fi, airm0, mask, layout, bbox, sum_mode=1):
+def flux_masses(fo, fi, airm0, mask, layout, bbox, sum_mode=1, autobar=False):
     """mass_tr_PM.F90:101-127: masses of FLUX_out and FLUX_in over Adz_i0b:Adz_inb x
-    Adz_j0b:Adz_jnb x 1:nk with the air mass at time M.  Returns [mass_fo, mass_fi] per
-    tracer, flattened."""
+    Adz_j0b:Adz_jnb x 1:nk with the air mass at time M (autobar: level 1 with mask =
+    geomh_area_mask_8, :105-112).  Returns [mass_fo, mass_fi] per tracer, flattened."""
     i0b, inb, j0b, jnb = bbox
     scope = (i0b, inb, j0b, jnb, 1)
     out = []
     for o, i_ in zip(fo, fi):
-        out += [mass(o, airm0, mask, layout, scope, sum_mode), mass(i_, airm0, mask, layout, scope, sum_mode)]
+        out += [mass(o, airm0, mask, layout, scope, sum_mode, autobar),
+                mass(i_, airm0, mask, layout, scope, sum_mode, autobar)]
     return np.array(out)
 
 

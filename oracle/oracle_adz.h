@@ -571,6 +571,7 @@ struct BCDomain {
   int min_max;                      // Adz_BC_min_max_L
   const float* pexp_list;           // Adz_BC_pexp_list (9)
   int sum_mode;                     // 0 sequential (mass_tr_*), 1 exactly rounded
+  int autobar = 0;                  // Schm_autobar_L: `mask` is geomh_area_mask_8
 };
 
 template <class T, class A>
@@ -579,6 +580,17 @@ inline double mass_of(const BCDomain& D, const T* x, const A* air) {
   // real) then * geomh_mask_8.
   std::vector<double> terms;
   double acc = 0.0;
+  if (D.autobar) {
+    // mass_tr_X.F90:73-79 (and mass_tr_PM.F90:79-85): level 1 only, tracer (real, promoted)
+    // times geomh_area_mask_8, no air mass
+    for (int j = D.j0; j <= D.jn; j++)
+      for (int i = D.i0; i <= D.in; i++) {
+        long q2 = (long)(i - D.minx) + (long)(D.maxx - D.minx + 1) * (j - D.miny);
+        double term = (double)x[q2] * D.mask[q2];
+        if (D.sum_mode == 0) acc = acc + term; else terms.push_back(term);
+      }
+    return D.sum_mode == 0 ? acc : fsum(terms);
+  }
   for (int k = D.k0; k <= D.nk; k++)
     for (int j = D.j0; j <= D.jn; j++)
       for (int i = D.i0; i <= D.in; i++) {

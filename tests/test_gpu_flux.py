@@ -61,10 +61,11 @@ def _core(W, kc0):
     return (1 + p, W["ni"] - p, 1 + p, W["nj"] - p, kc0)
 
 
-def _oracle_flux(W, trs, core, air):
+def _oracle_flux(W, trs, core, air, mask=None, autobar=False):
     zt = O.Ztab(W["g"]["zt"][1:W["nk"] + 1], W["nx"], W["ny"], 0, 0)
     fo, fi = O.bc_lam_aranami(trs, W["layout"], (W["ni"], W["nj"]), W["bbox"], core, zt, W["pb"].cpu().numpy())
-    masses = O.flux_masses(fo, fi, air.cpu().numpy(), W["mask"].cpu().numpy(), W["layout"], W["bbox"])
+    mk = (W["mask"] if mask is None else mask).cpu().numpy()
+    masses = O.flux_masses(fo, fi, air.cpu().numpy(), mk, W["layout"], W["bbox"], autobar=autobar)
     return fo, fi, masses
 
 
@@ -102,9 +103,13 @@ def test_flux_arrays_and_masses(W, dt, kc0):
     fx.free()
 
 
-def test_fixer_with_flux_bit_exact(W):
+@pytest.mark.parametrize("autobar", [False, True])
+def test_fixer_with_flux_bit_exact(W, autobar):
+    """the fixer with the flux against the oracle; with Schm_autobar_L every mass is the
+    level-1 sum weighted by geomh_area_mask_8 (mass_tr_PM.F90:79-85,105-112,
+    mass_tr_X.F90:73-79), given here as the mask"""
     torch, A = W["torch"], W["A"]
-    core = _core(W, W["k0t"])
+    core = _core(W, 1 if autobar else W["k0t"])
     rng = np.random.default_rng(41)
     trs = []
     for t in W["tr"]:
@@ -116,20 +121,24 @@ def test_fixer_with_flux_bit_exact(W):
     dev = [{k: (torch.from_numpy(v.copy()).cuda() if isinstance(v, np.ndarray) else v) for k, v in t.items()}
            for t in trs]
     fx = A.BcLamFlux([dict(src=d["p"]) for d in dev], W["layout"], (W["ni"], W["nj"]), W["bbox"], core, W["off"],
-                     W["ctx"].ztab(True), pos_dtype=W["pdt"])
-    bc = A.BermejoConde(dev, W["layout"], core, W["off"], min_max=True)
+                     W["ctx"].ztab(True), pos_dtype=W["pdt"], autobar=autobar)
+    bc = A.BermejoConde(dev, W["layout"], core, W["off"], min_max=True, autobar=autobar)
     mass = torch.zeros(2 * len(trs), dtype=torch.float64, device="cuda")
     deficit = torch.tensor([0.0, 3e-12, -2e-12], dtype=torch.float64, device="cuda")
     diag = torch.zeros(6 * len(trs), dtype=torch.float64, device="cuda")
     host_def = deficit.cpu().numpy()
-    a1, a0, mk = (x.cpu().numpy() for x in (W["airm1"], W["airm0"], W["mask"]))
-    _, _, hmass = _oracle_flux(W, [t["p"] for t in trs], core, W["airm0"])
+    # geomh_area_mask_8 under autobar: cell areas (m2) times the mask
+    wmask = W["mask"] * torch.linspace(1.1e8, 0.9e8, W["mask"].numel(), dtype=torch.float64, device="cuda") \
+        if autobar else W["mask"]
+    a1, a0, mk = (x.cpu().numpy() for x in (W["airm1"], W["airm0"], wmask))
+    _, _, hmass = _oracle_flux(W, [t["p"] for t in trs], core, W["airm0"], wmask, autobar)
     for call in range(2):
-        fx(W["pb"], W["airm0"], W["mask"], mass)
-        bc(W["airm1"], W["airm0"], W["mask"], deficit, diag=diag, flux=mass)
+        fx(W["pb"], W["airm0"], wmask, mass)
+        bc(W["airm1"], W["airm0"], wmask, deficit, diag=diag, flux=mass)
         torch.cuda.synchronize()
         assert np.array_equal(_bits(mass.cpu().numpy()), _bits(hmass)), call
-        host_def, hdiag = O.bermejo_conde(trs, W["layout"], core, a1, a0, mk, host_def, flux=hmass)
+        host_def, hdiag = O.bermejo_conde(trs, W["layout"], core, a1, a0, mk, host_def, flux=hmass,
+                                          autobar=autobar)
         for n, t in enumerate(trs):
             got = dev[n]["m"].cpu().numpy()
             assert np.array_equal(_bits(got), _bits(t["m"])), (call, n, int(np.count_nonzero(got != t["m"])))

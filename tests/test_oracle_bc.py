@@ -194,3 +194,41 @@ def test_correction_shape_is_the_weight(weight, pexp_n, deficit_sign):
     if weight == 1:
         flipped = _expected_weights(before, t["lin"], -D, float(pexp_n), 1, pkps)
         assert not _shape_matches(corr, flipped[sl], scale)   # sign rule reversed
+
+
+@pytest.mark.parametrize("min_max", [False, True])
+def test_autobar_masses_are_level_one_area_sums(min_max):
+    """Schm_autobar_L (mass_tr_PM.F90:79-85, mass_tr_X.F90:73-79): every mass is the sum over
+    level 1 of tracer * geomh_area_mask_8, with no air mass.  Checked by its consequences:
+    the air masses and the levels above 1 do not change any mass (scrambling them leaves
+    the diagnostics unchanged), the fixer restores the target mass measured that way, the
+    carried-deficit identity holds, and the target is the level-1 area sum of p minus the
+    deficit (a scope or weight taken from the 3-D form fails these)."""
+    rng = np.random.default_rng(29)
+    layout, scope, a1, a0, mask, trs = _case(rng, np.float64)
+    area = rng.uniform(1e6, 4e6, mask.shape)
+    base = [{k: (v.copy() if isinstance(v, np.ndarray) else v) for k, v in t.items()} for t in trs]
+    de, diag = O.bermejo_conde(trs, layout, scope, a1, a0, area, np.full(len(trs), 2e-9), min_max=min_max,
+                               autobar=True)
+    nk = layout[4]
+    level1 = lambda x: x.reshape(nk, -1)[0].reshape(mask.shape)
+    i0, in_, j0, jn, _ = scope
+    sl = (slice(j0 - layout[2], jn - layout[2] + 1), slice(i0 - layout[0], in_ - layout[0] + 1))
+    for n, t in enumerate(trs):
+        target = diag[n, 0]
+        assert target == O.fsum((level1(t["p"]) * area)[sl].ravel()) - 2e-9
+        got = O.fsum((level1(t["m"]) * area)[sl].ravel())
+        assert abs(got - target) <= 1e-13 * abs(target)
+        assert de[n] == got - target
+    # air masses and the upper levels' values of p do not enter any mass
+    trs2 = [{k: (v.copy() if isinstance(v, np.ndarray) else v) for k, v in t.items()} for t in base]
+    for t in trs2:
+        t["p"].reshape(nk, -1)[1:] *= 3.0
+    de2, diag2 = O.bermejo_conde(trs2, layout, scope, a1 * 2.0, a0 * 0.5, area, np.full(len(trs), 2e-9),
+                                 min_max=min_max, autobar=True)
+    assert np.array_equal(diag2[:, 0], diag[:, 0]) and np.array_equal(diag2[:, 1], diag[:, 1])
+    # the 3-D form of the same inputs gives other masses
+    _, diag3 = O.bermejo_conde([{k: (v.copy() if isinstance(v, np.ndarray) else v) for k, v in t.items()}
+                                for t in base], layout, scope, a1, a0, area, np.full(len(trs), 2e-9),
+                               min_max=min_max)
+    assert not np.array_equal(diag3[:, 0], diag[:, 0])
