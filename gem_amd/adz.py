@@ -328,6 +328,7 @@ L.extra_sigs({
     "adz_bermejo_conde": ("i", ["vp"] * 7 + ["vp"]),
     "adz_bermejo_conde_lam": ("i", ["vp"] * 8 + ["vp"]),
     "adz_bc_free": ("i", ["vp"]),
+    "adz_bermejo_conde_static": ("i", ["vp", "i", "vp", "i"] + ["vp"] * 7 + ["vp"]),
     "adz_flux_create": ("i", ["vp", "i", "vp", "vp", "i", "i", "vp"]),
     "adz_bc_lam_flux": ("i", ["vp"] * 5 + ["vp"]),
     "adz_flux_active": ("i", ["vp", "vp"]),

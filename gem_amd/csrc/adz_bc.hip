@@ -18,6 +18,7 @@
 // tracer.  Scalars stay on the device, so a timestep can be captured in a hipGraph.
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -432,6 +433,100 @@ int adz_bermejo_conde_lam(adz_bc* bc, const void* airm1, const void* airm0, cons
                           const double* flux_mass, double* deficit, double* diag, adz_stream stream) {
   if (!flux_mass) return fail(ADZ_EINVAL, "adz_bermejo_conde_lam: null flux masses");
   return bc_apply(bc, airm1, airm0, pkps, mask, flux_mass, deficit, diag, stream);
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------------
+// The fixer without allocation.  The tracer table, the block partials and the per-tracer
+// scalars of one call live in `scratch`, a file-scope device array of fixed size that each
+// GPU holds for the life of the process; the host reaches it through the address that
+// hipGetSymbolAddress gives on that device (the array's name in host code is only a
+// stand-in).  Calls that overlap on one GPU share the array and race.
+namespace {
+
+constexpr int kBcStaticTracers = 64;
+constexpr size_t kBcStaticTb = (sizeof(BcTr) * kBcStaticTracers + 255) / 256 * 256;
+constexpr size_t kBcStaticPb = sizeof(double) * 8 * kBcStaticTracers * kBcMaxBlocks;
+constexpr size_t kBcStaticSb = sizeof(double) * kSc * kBcStaticTracers;
+constexpr size_t kBcScratchBytes = kBcStaticTb + kBcStaticPb + kBcStaticSb;  // 8.4 MB
+constexpr int kFillChunk = 16;
+constexpr int kMaxDevices = 64;
+
+struct BcTrChunk {
+  BcTr t[kFillChunk];
+  int n0, cnt;
+};
+
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kBcScratchBytes];
+
+namespace {
+
+// tracers n0 .. n0+cnt-1 into the table; their scalars start at zero
+__global__ __launch_bounds__(256) void k_bc_static_fill(BcTrChunk c, BcTr* table, double* sc) {
+  const int l = threadIdx.x;
+  if (l < c.cnt) table[c.n0 + l] = c.t[l];
+  for (int q = l; q < c.cnt * kSc; q += 256) sc[(long)c.n0 * kSc + q] = 0.0;
+}
+
+std::mutex g_scratch_mu;
+void* g_scratch[kMaxDevices];
+
+// the device address of `scratch` on the current device (looked up once per device)
+int scratch_address(void** out) {
+  int dev = 0;
+  ADZ_HIP_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail(ADZ_EINVAL, "adz_bermejo_conde_static: device index too large");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch[dev]) {
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(ADZ_EHIP, std::string("adz_bermejo_conde_static: hipGetSymbolAddress: ") + hipGetErrorString(e));
+    g_scratch[dev] = p;
+  }
+  *out = g_scratch[dev];
+  return ADZ_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int adz_bermejo_conde_static(const adz_bc_tracer* tr, int ntr, const adz_bc_domain* dom, int tdtype,
+                             const void* airm1, const void* airm0, const void* pkps, const double* mask,
+                             const double* flux_mass, double* deficit, double* diag, adz_stream stream) {
+  if (!tr || ntr <= 0 || !dom) return fail(ADZ_EINVAL, "adz_bermejo_conde_static: bad argument");
+  if (ntr > kBcStaticTracers)
+    return fail(ADZ_EINVAL, "adz_bermejo_conde_static: at most 64 tracers per call (the scratch array's size)");
+  if (tdtype != ADZ_F32 && tdtype != ADZ_F64)
+    return fail(ADZ_EINVAL, "adz_bermejo_conde_static: tdtype must be F32 or F64");
+  BcTr h[kBcStaticTracers];
+  BcArgs a;
+  int has_w3 = 0;
+  if (int rc = bc_setup(tr, ntr, dom, "adz_bermejo_conde_static", h, &a, &has_w3)) return rc;
+  if (!airm1 || !airm0 || !mask || !deficit) return fail(ADZ_EINVAL, "adz_bermejo_conde_static: null argument");
+  if (has_w3 && !pkps) return fail(ADZ_EINVAL, "adz_bermejo_conde_static: weight 3 needs pkps");
+  void* base = nullptr;
+  if (int rc = scratch_address(&base)) return rc;
+  char* p = (char*)base;
+  BcTr* table = (BcTr*)p;
+  a.tr = table;
+  a.part = (double*)(p + kBcStaticTb);
+  a.sc = (double*)(p + kBcStaticTb + kBcStaticPb);
+  hipStream_t st = (hipStream_t)stream;
+  for (int n0 = 0; n0 < ntr; n0 += kFillChunk) {
+    BcTrChunk c;
+    std::memset(&c, 0, sizeof(c));
+    c.n0 = n0;
+    c.cnt = std::min(kFillChunk, ntr - n0);
+    for (int q = 0; q < c.cnt; q++) c.t[q] = h[n0 + q];
+    hipLaunchKernelGGL(k_bc_static_fill, dim3(1), dim3(256), 0, st, c, table, a.sc);
+  }
+  ADZ_HIP_CHECK(hipGetLastError());
+  return bc_launch(a, tdtype, has_w3, airm1, airm0, pkps, mask, flux_mass, deficit, diag, stream);
 }
 
 int adz_bc_free(adz_bc* bc) {

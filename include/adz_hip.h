@@ -323,6 +323,17 @@ int adz_bermejo_conde(adz_bc* bc, const void* airm1, const void* airm0, const vo
 int adz_bermejo_conde_lam(adz_bc* bc, const void* airm1, const void* airm0, const void* pkps, const double* mask,
                           const double* flux_mass, double* deficit, double* diag, adz_stream stream);
 int adz_bc_free(adz_bc* bc);
+/* The fixer in one call that allocates nothing: adz_bc_create + adz_bermejo_conde (flux_mass
+ * NULL) or adz_bermejo_conde_lam (flux_mass given) + adz_bc_free, with the same arguments
+ * and bit for bit the same results.  No counterpart in the reference: it is for callers
+ * that cannot allocate device memory (code under hipGraph capture, tight serving loops).
+ * The tracer table, block partials and per-tracer scalars live in a fixed device array of
+ * 8.4 MB that each GPU holds until the process ends; at most 64 tracers per call (more
+ * return ADZ_EINVAL).  Every call on one GPU uses that GPU's array: calls that overlap on
+ * one GPU (on different streams, or in graphs replayed concurrently) race. */
+int adz_bermejo_conde_static(const adz_bc_tracer* tr, int ntr, const adz_bc_domain* dom, int tdtype,
+                             const void* airm1, const void* airm0, const void* pkps, const double* mask,
+                             const double* flux_mass, double* deficit, double* diag, adz_stream stream);
 
 /* ------------------------------------------------------------------------------------
  * Level 4b: LAM boundary fluxes of the fixer, Aranami et al. (2015), Adz_BC_LAM_flux = 1:
