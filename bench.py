@@ -63,6 +63,12 @@ def parse():
     p.add_argument("--fortran", action="store_true",
                    help="c5 through the Fortran drop-in layer instead (module adz_gpu: adz_main + adz_tracers "
                         "before and after the pressure adjustment, one hipGraph), timed alone")
+    p.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
+                   help="write profiles/k_plan_<config>_pmc.json from two rocprofv3 --pmc runs of "
+                        "--profile-kernel (FETCH_SIZE, then WRITE_SIZE; their csv output folders) and exit")
+    p.add_argument("--kernel-trace", metavar="KT_DIR",
+                   help="with --pmc-summary: the rocprofv3 --kernel-trace --stats folder of the same command, "
+                        "for the average launch duration")
     p.add_argument("--retune", action="store_true",
                    help="run adz_plan_autotune instead of using the tiles it found before (gem_amd/tuned_tiles.json)")
     return p.parse_args()
@@ -82,6 +88,8 @@ def main():
     from gem_amd import adz as A
     from gem_amd import synth as S
 
+    if args.pmc_summary:
+        return pmc_summary(args)
     assert torch.cuda.is_available(), "bench.py needs a HIP device"
     dev = torch.device("cuda:0")
     if args.config == "c4":
@@ -492,8 +500,8 @@ def main():
         "error_vs_fp64": {"worst_max_rel": worst_max, "worst_rms_rel": worst_rms, "per_field_max_rms": errs},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK, 4), "traffic": traffic,
-                     "kernel": f"k_plan<float,{'float' if pos_dt == L.ADZ_F32 else 'double'},F32,post> tracer group "
-                               f"({ng} fields), "
+                     "kernel": f"{plan_kernel_name(tuneB, 'float' if pos_dt == L.ADZ_F32 else 'double', 'post')} "
+                               f"tracer group ({ng} fields), "
                                f"{t_plan0:.3f} ms/launch, {alg_plan0 / 1e9:.2f} GB algorithmic"},
         "cpu_baseline": cpu,
     }
@@ -754,13 +762,73 @@ def run_c4(args, np, torch, L, A, S, dev):
         "error_vs_fp64": {"worst_max_rel": worst_max, "worst_rms_rel": worst_rms, "fields": nerr},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK, 4), "traffic": None,
-                     "kernel": f"k_plan<float,double,F32,0> over 16 items, {t_plan:.3f} ms/launch, "
+                     "kernel": f"{plan_kernel_name(tune, 'double', '0')} over 16 items, {t_plan:.3f} ms/launch, "
                                f"{alg / 1e9:.2f} GB algorithmic"},
         "cpu_baseline": None,
     }
     print(json.dumps(out), flush=True)
     plan.free()
     default_plan.free()
+
+
+def pmc_summary(args):
+    """profiles/k_plan_<config>_pmc.json: HBM bytes per launch of the tracer-group kernel
+    from rocprofv3 FETCH_SIZE and WRITE_SIZE runs of --profile-kernel, each counter scaled by
+    its reading for the run's closing 1 GiB device copy (1 GiB read, 1 GiB written)."""
+    import csv
+    import glob
+
+    def counters(folder, name):
+        (path,) = glob.glob(os.path.join(folder, "*counter_collection.csv"))
+        per = {}
+        for r in csv.DictReader(open(path)):
+            if r["Counter_Name"] != name:
+                continue
+            d = per.setdefault(int(r["Dispatch_Id"]), [r["Kernel_Name"], 0.0, int(r["Start_Timestamp"]),
+                                                       int(r["End_Timestamp"])])
+            d[1] += float(r["Counter_Value"])
+        plan = [v for v in per.values() if "k_plan" in v[0]]
+        copy = [v for v in per.values() if "copy" in v[0].lower()]
+        assert plan and copy, f"{path}: no k_plan or copy dispatch"
+        return plan, copy[-1]
+
+    fplan, fcopy = counters(args.pmc_summary[0], "FETCH_SIZE")
+    wplan, wcopy = counters(args.pmc_summary[1], "WRITE_SIZE")
+    kib = 2 ** 30 / 1024  # the copy moves 1 GiB each way
+    fscale, wscale = kib / fcopy[1], kib / wcopy[1]
+    fk = sum(v[1] for v in fplan) / len(fplan)
+    wk = sum(v[1] for v in wplan) / len(wplan)
+    dur = sum(v[3] - v[2] for v in fplan) / len(fplan)
+    if args.kernel_trace:
+        (st,) = glob.glob(os.path.join(args.kernel_trace, "*kernel_stats.csv"))
+        for r in csv.DictReader(open(st)):
+            if "k_plan" in r["Name"]:
+                dur = float(r["AverageNs"])
+    tiles = (tuned_tiles(args.config, args.tiles) or {}).get("tracer_groups")
+    out = {"kernel": fplan[0][0],
+           "workload": "c5 tracer group: 20 FP32 tracers (model layout 2052x1028x84) at the thermodynamic departure "
+                       "points of 2048x1024x84 targets, lin/min/max stored",
+           "tiles": tiles, "kernel_source_sha16": kernel_source_sha(),
+           "fetch_size_kib_per_launch": fk, "write_size_kib_per_launch": wk,
+           "copy_fetch_size_kib": fcopy[1], "copy_write_size_kib": wcopy[1],
+           "fetch_scale": fscale, "write_scale": wscale,
+           "hbm_read_bytes_per_launch": int(fk * fscale * 1024), "hbm_write_bytes_per_launch": int(wk * wscale * 1024),
+           "hbm_bytes_per_launch": int((fk * fscale + wk * wscale) * 1024),
+           "algorithmic_bytes_per_launch": 73.1e9, "average_launch_ns": dur}
+    path = os.path.join(ROOT, "profiles", f"k_plan_{args.config}_pmc.json")
+    with open(path, "w") as f:
+        json.dump(out, f, indent=1)
+    print(json.dumps({"pmc_summary": path, "hbm_bytes_per_launch": out["hbm_bytes_per_launch"]}))
+
+
+def plan_kernel_name(tune, pos, post):
+    """the batched kernel a plan with these tiles launches for FP32 fields and arithmetic"""
+    stage = (tune or {}).get("stage") or 5
+    if stage == 5:
+        return f"k_plan5<{pos},F32,{post}>"
+    if stage == 3:
+        return f"k_plan_pc<{pos},F32,{post}>"
+    return f"k_plan<float,{pos},F32,{post},stage {stage}>"
 
 
 def tuned_tiles(config, path):

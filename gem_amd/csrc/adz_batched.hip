@@ -199,6 +199,18 @@ __device__ __forceinline__ void block_min6(int* v, int* red) {
   __syncthreads();
 }
 
+// The outputs of one target (value, and with POST lin / min / max) from a stencil accessor.
+template <class TF, int MATH, int POST, class Acc, class TC>
+__device__ __forceinline__ void eval_vals(Acc at, const void* cw, TC* v, TC* l, TC* mi, TC* ma) {
+  if constexpr (MATH == ADZ_MATH_REF) {
+    *v = tricub_ref_a<TF>(at, (const double*)cw);
+    if (POST) mono_extra_a<TF, double>(at, (const double*)cw, l, mi, ma);
+  } else {
+    *v = tricub_f32_a<TF>(at, (const float*)cw);
+    if (POST) mono_extra_a<TF, float>(at, (const float*)cw, l, mi, ma);
+  }
+}
+
 // Per-lane evaluation of fields [f0, f1) for one target: every lane reads its own 16
 // stencil rows (buffer loads for FP32 sources, global loads otherwise).
 template <class TF, int MATH, int POST, class Fields>
@@ -250,24 +262,33 @@ __device__ __forceinline__ void per_lane(const DevItem& it, const Fields& F, int
 //     (within the super-column), which keeps the blocks resident on one XCD inside a
 //     working set its L2 can hold.  Tiles past the last row (njb not a multiple of S) hold
 //     no valid target.
-__device__ __forceinline__ void tile_coords(const DevItem& it, long ptile, int level_inner, int* ib, int* jb,
-                                            int* kk, int tw = kTI) {
+struct TileXY {
+  int ib, jb, kk;
+};
+__device__ __forceinline__ TileXY tile_at(const DevItem& it, long ptile, int level_inner, int tw = kTI) {
   const long nib = (it.nx + tw - 1) / tw, njb = (it.ny + kTJ - 1) / kTJ;
+  TileXY c;
   if (level_inner >= 2) {
     const long S = level_inner, per = (long)it.nz * S;
     const long sc = ptile / per, r = ptile % per;
-    *kk = (int)(r / S);
-    *ib = (int)(sc % nib);
-    *jb = (int)((sc / nib) * S + r % S);
+    c.kk = (int)(r / S);
+    c.ib = (int)(sc % nib);
+    c.jb = (int)((sc / nib) * S + r % S);
   } else if (level_inner) {
-    *kk = (int)(ptile % it.nz);
-    *ib = (int)((ptile / it.nz) % nib);
-    *jb = (int)(ptile / (it.nz * nib));
+    c.kk = (int)(ptile % it.nz);
+    c.ib = (int)((ptile / it.nz) % nib);
+    c.jb = (int)(ptile / (it.nz * nib));
   } else {
-    *ib = (int)(ptile % nib);
-    *jb = (int)((ptile / nib) % njb);
-    *kk = (int)(ptile / (nib * njb));
+    c.ib = (int)(ptile % nib);
+    c.jb = (int)((ptile / nib) % njb);
+    c.kk = (int)(ptile / (nib * njb));
   }
+  return c;
+}
+__device__ __forceinline__ void tile_coords(const DevItem& it, long ptile, int level_inner, int* ib, int* jb,
+                                            int* kk, int tw = kTI) {
+  const TileXY c = tile_at(it, ptile, level_inner, tw);
+  *ib = c.ib; *jb = c.jb; *kk = c.kk;
 }
 
 // tiles of an item in the order above
@@ -830,6 +851,155 @@ __device__ __forceinline__ void shared_tile(const DevItem& it, const Fields& F, 
   if (slow_any) per_lane<float, MATH, POST>(it, F, f0, f1, x, target && ok && !fast, false, cw, o);
 }
 
+// ---------------------------------------------------------------------------------
+// Staging mode 5 (FP32 sources): the tile's box (the union of its 256 stencils) is copied
+// into LDS once per field by all four waves with 16-byte buffer-to-LDS DMA, and every lane
+// then reads its 64 stencil values from LDS.  Per-lane loads spend 16 address slots of the
+// texture unit per point-field; the box needs about 2 (a 68 x 7 x 4 box is 28 rows of 18
+// quads for 256 targets).  LDS rows have a fixed pitch of 20 quads (80 floats): the rows of
+// a buffer are dense, each DMA instruction fills 64 consecutive quads of them, and the 16
+// values of one stencil plane lie within 1 KB of one per-lane base, so they are read in
+// pairs at immediate offsets.  The DMA starts each row at a 16-byte boundary, so a field
+// whose base is not 16-byte aligned lands shifted by `sh` floats (0..3), added to the
+// bases.  Two LDS buffers with static roles: field f+1 is in flight while field f is
+// evaluated; one block barrier per field.  Tiles whose box does not fit (more than
+// kMaxRows5 rows or 77 columns), rows not a multiple of 4 floats apart, or sources beyond
+// 32-bit buffer offsets use per-lane loads.  The arithmetic and its order are those of the
+// per-lane path, so both give the same bits.
+constexpr int kQRow = 20;                               // 16-byte quads per LDS row
+constexpr int kPitch5 = 4 * kQRow;                      // floats per LDS row
+constexpr int kMaxRows5 = kMaxRows * kPitch / kPitch5;  // rows per buffer (the stage-1 buffers)
+constexpr int kDmaMax = 5;                              // DMA instructions per wave and field
+static_assert(4 * kDmaMax * 64 >= kMaxRows5 * kQRow, "DMA slots cover a full buffer");
+
+template <int MATH, int POST, class TP, class Fields>
+__device__ __forceinline__ void dma_tile(const DevItem& it, const Fields& F, long ptile, int f0, int f1,
+                                         float* boxA, float* boxB, int* red, int level_inner) {
+  const int lane = threadIdx.x & (kTI - 1), wv = threadIdx.x / kTI;
+  const TileXY tc = tile_at(it, ptile, level_inner);
+  const int ii = tc.ib * kTI + lane, jj = tc.jb * kTJ + wv;
+  const bool valid = ii < it.nx && jj < it.ny;
+  const int i = it.i0 + ii, j = it.j0 + jj, k = it.k0 + tc.kk;
+  const long p = (long)ii + (long)it.nx * (jj + (long)it.ny * tc.kk);
+  const long o = (long)(i - 1) + (long)it.dni * ((j - 1) + (long)it.dnj * (k - 1));
+  Index x;
+  bool ok = false;
+  if (valid) {
+    double px, py, pz;
+    load_pos<TP>(it.xyz, p, &px, &py, &pz);
+    ok = index_step(px, py, pz, it.lv, &x);
+  }
+  double cw[20];
+  if (ok) {
+    if (MATH == ADZ_MATH_REF) weights_ref(x, it.lv, cw);
+    else weights_f32(x, it.lv, (float*)cw);
+  }
+  const int big = 1 << 30;
+  int ext[6] = {ok ? x.ix - 1 : big, ok ? -(x.ix + 2) : big, ok ? x.iy - 1 : big, ok ? -(x.iy + 2) : big,
+                ok ? x.pz : big, ok ? -(x.pz + (x.zlinear ? 1 : 3)) : big};
+  block_min6(ext, red);
+#pragma unroll
+  for (int q = 0; q < 6; q++) ext[q] = __builtin_amdgcn_readfirstlane(ext[q]);  // block-uniform
+  const int x0 = ext[0], y0 = ext[2], z0 = ext[4];
+  if (x0 == big) {  // no target of the tile has an in-range stencil
+    if (valid)
+      for (int f = f0; f < f1; f++) store_nan<float, MATH, POST>(F, f, o);
+    return;
+  }
+  const int nxb = -ext[1] - x0 + 1, nyb = -ext[3] - y0 + 1, nzb = -ext[5] - z0 + 1;
+  const int nrows = nyb * nzb, cpr = (nxb + 3 + 3) / 4;  // quads per row, room for the shift
+  const long gni = it.lv.ni, gnij = it.lv.nij;
+  // every quad the DMA reads, for any shift 0..3, inside the buffer's range [minoff,
+  // minoff + span): a quad reaching outside would read zeros for its in-range floats too
+  const long first = (long)(x0 - 1 - 3) + (long)(y0 - 1) * gni + (long)z0 * gnij - it.minoff;
+  const long last = (long)(x0 - 1) + 4L * cpr - 1 + (long)(y0 + nyb - 2) * gni + (long)(z0 + nzb - 1) * gnij -
+                    it.minoff;
+  if (nrows > kMaxRows5 || cpr > kQRow || !it.span || (gni & 3) || (gnij & 3) || first < 0 ||
+      (last + 1) * 4 > (long)it.span) {
+    per_lane<float, MATH, POST>(it, F, f0, f1, x, ok, valid, cw, o);
+    return;
+  }
+  // this lane's DMA slots: instruction q of wave wv fills the 64 quads g*64 .. g*64+63 of
+  // the buffer (g = wv + 4 q), quad Q = row Q / kQRow, column quad Q % kQRow; element
+  // offsets from minoff of the lane's quad (before the field's shift)
+  // (32-bit arithmetic: every offset is below span < 2^31 bytes; R / nyb by a float
+  // reciprocal, exact since R < 64 and nyb < 64 keep (R + 0.5) / nyb >= 1/128 from an integer)
+  const int rel0 = (int)__builtin_amdgcn_readfirstlane((int)((long)(x0 - 1) + (long)(y0 - 1) * gni +
+                                                             (long)z0 * gnij - it.minoff));
+  const int ni32 = (int)gni, nij32 = (int)gnij;
+  const float rnyb = 1.0f / (float)nyb;
+  int rel[kDmaMax];
+  unsigned act = 0;
+#pragma unroll
+  for (int q = 0; q < kDmaMax; q++) {
+    const int Q = (wv + 4 * q) * 64 + lane;
+    const int R = Q / kQRow, cq = Q - R * kQRow;
+    const int zz = (int)(((float)R + 0.5f) * rnyb), yy = R - zz * nyb;
+    rel[q] = rel0 + 4 * cq + yy * ni32 + zz * nij32;
+    if (R < nrows && cq < cpr) act |= 1u << q;
+  }
+  // plane bases of this lane's stencil in a buffer (floats, before the field's shift)
+  int lb[4] = {0, 0, 0, 0};
+  if (ok) {
+    const int r0 = x.iy - 1 - y0, pz0 = x.pz - z0, c0 = x.ix - 1 - x0;
+#pragma unroll
+    for (int z = 0; z < 4; z++) {
+      const int dz = x.zlinear ? (z >> 1) : z;
+      lb[z] = (r0 + nyb * (pz0 + dz)) * kPitch5 + c0;
+    }
+  }
+  auto shift = [&](int f) {  // floats between the field's 16-byte boundary and column x0
+    const unsigned long long b = (unsigned long long)F.s(f);
+    return (int)(((b >> 2) + (unsigned)(x0 - 1)) & 3);
+  };
+  auto issue = [&](int f, float* buf) {
+    const int sh = shift(f);
+    const __amdgpu_buffer_rsrc_t rs = field_rsrc((const float*)F.s(f) + it.minoff, it.span);
+#pragma unroll
+    for (int q = 0; q < kDmaMax; q++) {
+      if (act & (1u << q))
+        __builtin_amdgcn_raw_ptr_buffer_load_lds(
+            rs, (__attribute__((address_space(3))) void*)(buf + (wv + 4 * q) * 256), 16,
+            (rel[q] - sh) * 4, 0, 0, 0);
+    }
+  };
+  // Every lane evaluates (one without a stencil reads its buffer's first values and stores
+  // NaN where valid): with one store site the field index and output pointers stay scalar.
+  typedef typename std::conditional<MATH == ADZ_MATH_REF, double, float>::type TC;
+  auto eval = [&](int f, const float* buf) {
+    const int sh = shift(f);
+    const float* b0 = buf + lb[0] + sh;
+    const float* b1 = buf + lb[1] + sh;
+    const float* b2 = buf + lb[2] + sh;
+    const float* b3 = buf + lb[3] + sh;
+    auto at = [&](int r, int z, int c) -> float {
+      const float* bz = z == 0 ? b0 : (z == 1 ? b1 : (z == 2 ? b2 : b3));
+      return bz[r * kPitch5 + c];
+    };
+    TC v = 0, l = 0, mi = 0, ma = 0;
+    eval_vals<float, MATH, POST>(at, cw, &v, &l, &mi, &ma);
+    if (!ok) {
+      const TC nan = (TC)__builtin_nanf("");
+      v = nan; l = nan; mi = nan; ma = nan;
+    }
+    if (valid) store_out<float, MATH, POST>(F, f, o, v, l, mi, ma);
+  };
+  issue(f0, boxA);
+  asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+  __syncthreads();
+  for (int f = f0; f < f1; f += 2) {
+    if (f + 1 < f1) issue(f + 1, boxB);
+    eval(f, boxA);
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __syncthreads();
+    if (f + 1 >= f1) break;
+    if (f + 2 < f1) issue(f + 2, boxA);
+    eval(f + 1, boxB);
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __syncthreads();
+  }
+}
+
 // Persistent loop over the tile list of a plan.
 template <class TF, class TP, int MATH, int POST, int STAGE>
 __device__ __forceinline__ void plan_loop(const PlanArgs& A, TF* boxA, TF* boxB, int* red, int* rowoff) {
@@ -844,9 +1014,20 @@ __device__ __forceinline__ void plan_loop(const PlanArgs& A, TF* boxA, TF* boxB,
   }
   for (long t = t0; t < hi; t += step) {
     int q = 0;
-    const DevItem* items = A.items;  // a kernel argument: the compiler reads it with global loads
+    // The item table does not change during a launch: read through the constant address
+    // space, the (uniform) item's fields come in with scalar loads.
+    typedef const DevItem __attribute__((address_space(4))) CItem;
+    const CItem* items = (const CItem*)A.items;
     while (q + 1 < A.nitems && items[q + 1].tile0 <= t) q++;
-    const DevItem& it = items[q];
+    DevItem it;
+    {
+      static_assert(sizeof(DevItem) % 8 == 0, "DevItem is copied in 8-byte words");
+      typedef const unsigned long long __attribute__((address_space(4))) CWord;
+      const CWord* w = (const CWord*)(A.items + q);
+      unsigned long long* d = (unsigned long long*)&it;
+#pragma unroll
+      for (int e = 0; e < (int)(sizeof(DevItem) / 8); e++) d[e] = w[e];
+    }
     long local = t - it.tile0;
     long ptile, grp;
     if (A.points_outer) { ptile = local / it.ngroups; grp = local % it.ngroups; }
@@ -855,6 +1036,8 @@ __device__ __forceinline__ void plan_loop(const PlanArgs& A, TF* boxA, TF* boxB,
     ItemFields F{it.src, it.dst, it.lin, it.mn, it.mx};
     if constexpr (STAGE == 4 && std::is_same<TF, float>::value)
       shared_tile<MATH, POST, TP>(it, F, ptile, f0, f1, A.level_inner);
+    else if constexpr (STAGE == 5 && std::is_same<TF, float>::value)
+      dma_tile<MATH, POST, TP>(it, F, ptile, f0, f1, (float*)boxA, (float*)boxB, red, A.level_inner);
     else
       run_tile<TF, TP, MATH, POST, ItemFields>(it, F, ptile, f0, f1, boxA, boxB, red, rowoff, STAGE, A.level_inner);
   }
@@ -874,6 +1057,17 @@ __global__ __launch_bounds__(kTile) void k_plan(PlanArgs A) {
   } else {
     plan_loop<TF, TP, MATH, POST, STAGE>(A, nullptr, nullptr, nullptr, nullptr);
   }
+}
+
+// Staging mode 5 (FP32 sources): its own kernel.  In the FP32 arithmetic it is held to 128
+// VGPRs, so that four blocks per CU (the number its two LDS buffers allow) keep four waves
+// on every SIMD; the reference arithmetic needs more registers than that without spilling.
+template <class TP, int MATH, int POST>
+__global__ __launch_bounds__(kTile, MATH == ADZ_MATH_F32 ? 4 : 1) void k_plan5(PlanArgs A) {
+  __shared__ float boxA[kMaxRows * kPitch];
+  __shared__ float boxB[kMaxRows * kPitch];
+  __shared__ int red[6 * kTJ];
+  plan_loop<float, TP, MATH, POST, 5>(A, boxA, boxB, red, nullptr);
 }
 
 // One item with inline field pointers (the adz_tricub_hlt entry point).
@@ -938,6 +1132,14 @@ void* pick_plan(int post, int stage) {
         default: return (void*)&k_plan<TF, TP, M, 3, 4>;
       }
     }
+    if (stage == 5) {
+      switch (post) {
+        case 0: return (void*)&k_plan5<TP, M, 0>;
+        case 1: return (void*)&k_plan5<TP, M, 1>;
+        case 2: return (void*)&k_plan5<TP, M, 2>;
+        default: return (void*)&k_plan5<TP, M, 3>;
+      }
+    }
     if (stage == 3) {
       switch (post) {
         case 0: return (void*)&k_plan_pc<TP, M, 0>;
@@ -991,6 +1193,10 @@ void* pick(int fdt, int pdt, int math, int post, int stage = 2) {
 }
 
 // targets per tile along i: kSW for the shared-column staging of FP32 sources, else kTI
+// built-in tile parameters: LDS-staged boxes (stage 5; other field types fall back to
+// per-lane loads) over super-columns of three tile rows, four fields per pass
+adz_tune default_tune() { return adz_tune{4, 0, 4, 1, 5, 3}; }
+
 int tile_width(int stage, int fdtype) { return (stage == 4 && fdtype == ADZ_F32) ? kSW : kTI; }
 
 int device_cus() {
@@ -1046,17 +1252,19 @@ int adz_plan_create(const adz_item* items, int nitems, int fdtype, int math, int
   for (int q = 0; q < nitems; q++)
     if (items[q].pdtype != pdt) return fail(ADZ_EINVAL, "adz_plan_create: items must share the position dtype");
   if (!pick<true>(fdtype, pdt, math, post)) return fail(ADZ_EINVAL, "adz_plan_create: unsupported dtype/math");
-  if (tune && (tune->stage < 0 || tune->stage > 4)) return fail(ADZ_EINVAL, "adz_plan_create: stage must be 0..4");
+  if (tune && (tune->stage < 0 || tune->stage > 5)) return fail(ADZ_EINVAL, "adz_plan_create: stage must be 0..5");
   if (tune && (tune->xcd_chunks < 0 || tune->xcd_chunks > 1))
     return fail(ADZ_EINVAL, "adz_plan_create: xcd_chunks must be 0 or 1");
-  adz_tune T{8, 0, 4, 1, 2, 0};
+  adz_tune T = default_tune();
   if (tune) {
     if (tune->fields_per_pass > 0) T.fields_per_pass = std::min(64, tune->fields_per_pass);
     T.points_outer = tune->points_outer;
     if (tune->blocks_per_cu > 0) T.blocks_per_cu = tune->blocks_per_cu;
     T.xcd_chunks = tune->xcd_chunks;
-    if (tune->stage > 0) T.stage = tune->stage;
-    T.level_inner = tune->level_inner;
+    if (tune->stage > 0) {  // stage 0 keeps the built-in staging and its tile order
+      T.stage = tune->stage;
+      T.level_inner = tune->level_inner;
+    }
   }
   std::vector<DevItem> dv(nitems);
   long nptr_total = 0;
@@ -1151,10 +1359,11 @@ int adz_plan_tune(const adz_plan* p, adz_tune* out) {
   return ADZ_OK;
 }
 
-// Autotuner: a coordinate search over the tile parameters, each candidate timed with
-// HIP events on the caller's plan (its destinations are written, with the values any
-// execution writes).  Order: staging, fields per pass, persistent grid size, loop order,
-// XCD chunking; each coordinate keeps the best value found before moving on.
+// Autotuner: a search over the tile parameters, each candidate timed with HIP events on
+// the caller's plan (its destinations are written, with the values any execution writes).
+// Staging and tile order are searched jointly, then fields per pass, tile order again,
+// persistent grid size, loop order and XCD chunking one coordinate at a time; each step
+// keeps the best value found before moving on.
 int adz_plan_autotune(const adz_item* items, int nitems, int fdtype, int math, int post, int reps,
                       adz_stream stream, adz_tune* best, float* best_ms) {
   if (!best) return fail(ADZ_EINVAL, "adz_plan_autotune: null result");
@@ -1183,23 +1392,35 @@ int adz_plan_autotune(const adz_item* items, int nitems, int fdtype, int math, i
     int rf = adz_plan_free(p);
     return rc != ADZ_OK ? rc : rf;
   };
-  adz_tune cur{8, 0, 4, 1, 2, 0};
+  adz_tune cur = default_tune();
   cur.fields_per_pass = std::min(cur.fields_per_pass, maxf);
   float cur_ms = 0.f;
   int rc = time_one(cur, &cur_ms);
-  const int stages[] = {1, 2, 3, 4}, fpps[] = {2, 4, 8, 16, 32, 64}, bpcs[] = {2, 4, 8, 16}, flags[] = {0, 1},
-            orders[] = {0, 1, 2, 4, 8};
-  // coordinates: 0 staging, 1 tile order, 2 fields per pass, 3 blocks per CU, 4 loop order,
-  // 5 XCD chunking; the tile order is searched again after fields per pass (they interact
-  // through the L2 working set)
-  const int coords[] = {0, 1, 2, 1, 3, 4, 5};
-  for (int ci = 0; ci < 7 && rc == ADZ_OK; ci++) {
+  const int stages[] = {1, 2, 3, 4, 5}, fpps[] = {2, 4, 8, 16, 32, 64}, bpcs[] = {2, 4, 8, 16}, flags[] = {0, 1},
+            orders[] = {0, 1, 2, 3, 4, 6, 8};
+  const int nst = fdtype == ADZ_F32 ? 5 : 2, nor = 7;
+  // staging and tile order together first: the LDS-staged forms depend on the L2 reuse the
+  // tile order gives, so one is not searched at the other's best value
+  for (int s = 0; s < nst && rc == ADZ_OK; s++)
+    for (int v = 0; v < nor && rc == ADZ_OK; v++) {
+      adz_tune t = cur;
+      t.stage = stages[s];
+      t.level_inner = orders[v];
+      if (t.stage == cur.stage && t.level_inner == cur.level_inner) continue;
+      float ms = 0.f;
+      rc = time_one(t, &ms);
+      if (rc == ADZ_OK && ms < cur_ms) { cur = t; cur_ms = ms; }
+    }
+  // then coordinates: 2 fields per pass, 1 tile order again (they interact through the L2
+  // working set), 3 blocks per CU, 4 loop order, 5 XCD chunking
+  const int coords[] = {2, 1, 3, 4, 5};
+  for (int ci = 0; ci < 5 && rc == ADZ_OK; ci++) {
     const int coord = coords[ci];
-    const int* vals = coord == 0 ? stages : coord == 1 ? orders : coord == 2 ? fpps : coord == 3 ? bpcs : flags;
-    int nv = coord == 0 ? (fdtype == ADZ_F32 ? 4 : 2) : coord == 1 ? 5 : coord == 2 ? 6 : coord == 3 ? 4 : 2;
+    const int* vals = coord == 1 ? orders : coord == 2 ? fpps : coord == 3 ? bpcs : flags;
+    int nv = coord == 1 ? nor : coord == 2 ? 6 : coord == 3 ? 4 : 2;
     for (int v = 0; v < nv && rc == ADZ_OK; v++) {
       adz_tune t = cur;
-      int* slot = coord == 0 ? &t.stage : coord == 1 ? &t.level_inner : coord == 2 ? &t.fields_per_pass
+      int* slot = coord == 1 ? &t.level_inner : coord == 2 ? &t.fields_per_pass
                 : coord == 3 ? &t.blocks_per_cu : coord == 4 ? &t.points_outer : &t.xcd_chunks;
       if (*slot == vals[v]) continue;
       if (coord == 2 && vals[v] > maxf && vals[v - 1] >= maxf) continue;  // one candidate covers all fields

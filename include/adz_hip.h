@@ -226,8 +226,9 @@ typedef struct adz_item {
   int nptr;
 } adz_item;
 
-/* Tile parameters of the persistent kernel; zero fields take the built-in default
-   {8, 0, 4, 1, 2, 0}; adz_plan_autotune searches them on the actual plan. */
+/* Tile parameters of the persistent kernel; zero fields_per_pass / blocks_per_cu take the
+   built-in default {4, 0, 4, 1, 5, 3}, and stage 0 takes the built-in staging together
+   with its level_inner; adz_plan_autotune searches them on the actual plan. */
 typedef struct adz_tune {
   int fields_per_pass; /* fields interpolated per pass over the points (1..64)          */
   int points_outer;    /* 1: a block runs every field group of a point tile; 0: a pass
@@ -240,7 +241,10 @@ typedef struct adz_tune {
                           waves evaluate from LDS; 4 (FP32 sources): each lane loads one
                           stencil column, reduces it for its own and its three left
                           neighbours' targets, partials shared by lane shifts (61-wide
-                          tiles)                                                        */
+                          tiles); 5 (FP32 sources): all four waves copy each field's
+                          box into LDS with 16-byte DMA, double-buffered, and every
+                          lane evaluates from LDS (the fastest form at c5 with a
+                          level_inner of 3 to 6)                                        */
   int level_inner;     /* tile order. 0: i, j, then k; 1: consecutive tiles walk the
                           levels of one tile column; S >= 2: super-columns of S tile
                           rows walked level by level (source planes and rows reused

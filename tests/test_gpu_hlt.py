@@ -197,7 +197,10 @@ def _plan_case(T):
                                   dict(fields_per_pass=3, points_outer=0, blocks_per_cu=3, xcd_chunks=1, stage=3,
                                        level_inner=2),
                                   dict(fields_per_pass=2, points_outer=0, blocks_per_cu=4, xcd_chunks=1, stage=4,
-                                       level_inner=4)])
+                                       level_inner=4),
+                                  dict(fields_per_pass=5, points_outer=0, blocks_per_cu=4, xcd_chunks=1, stage=5),
+                                  dict(fields_per_pass=2, points_outer=1, blocks_per_cu=2, xcd_chunks=0, stage=5,
+                                       level_inner=3)])
 def test_plan_matches_single_calls(T, tune):
     torch, L, A = T
     items, refs, keep = _plan_case(T)
@@ -288,14 +291,16 @@ def test_shared_columns_match_single_calls(T, post, math, pdt):
         refs.append((stack, ref))
     for tune in (dict(fields_per_pass=2, points_outer=0, blocks_per_cu=4, xcd_chunks=1, stage=4),
                  dict(fields_per_pass=3, points_outer=1, blocks_per_cu=2, xcd_chunks=0, stage=4, level_inner=1),
-                 dict(fields_per_pass=1, points_outer=0, blocks_per_cu=3, xcd_chunks=1, stage=4, level_inner=3)):
+                 dict(fields_per_pass=1, points_outer=0, blocks_per_cu=3, xcd_chunks=1, stage=4, level_inner=3),
+                 dict(fields_per_pass=3, points_outer=0, blocks_per_cu=4, xcd_chunks=1, stage=5),
+                 dict(fields_per_pass=1, points_outer=1, blocks_per_cu=2, xcd_chunks=0, stage=5, level_inner=2)):
         for stack, _ in refs:
             for a in stack:
                 for k in ("dst", "lin", "min", "max"):
                     if k in a:
                         a[k].fill_(-7.0)
         plan = A.Plan(items, L.ADZ_F32, math=M, post=post, tune=tune)
-        assert plan.tune()["stage"] == 4
+        assert plan.tune()["stage"] == tune["stage"]
         plan.execute()
         torch.cuda.synchronize()
         for stack, ref in refs:
@@ -308,12 +313,14 @@ def test_shared_columns_match_single_calls(T, post, math, pdt):
         c.free()
 
 
+@pytest.mark.parametrize("stage", [4, 5])
 @pytest.mark.parametrize("math", ["f32", "ref"])
-def test_shared_columns_edge_positions(T, math):
-    """Staging mode 4 with positions that break the lane sharing on purpose: stencils out
-    of range (NaN targets whose lanes serve no neighbour), positions on cell faces, a cell
-    repeated or skipped every few targets, levels clamped at both ends.  Bit for bit the
-    single-call result (NaN patterns included), with lin/min/max."""
+def test_shared_columns_edge_positions(T, math, stage):
+    """Staging modes 4 and 5 with positions that break lane sharing and box staging on
+    purpose: stencils out of range (NaN targets whose lanes serve no neighbour), positions
+    on cell faces, a cell repeated or skipped every few targets, levels clamped at both
+    ends, stencils on the first columns of the source.  Bit for bit the single-call result
+    (NaN patterns included), with lin/min/max."""
     torch, L, A = T
     M = L.ADZ_MATH_F32 if math == "f32" else L.ADZ_MATH_REF
     cfg = S.Config("e", 150, 26, 10, 4, 300.0, seed=85)
@@ -334,6 +341,15 @@ def test_shared_columns_edge_positions(T, math):
     bad = rng.random((nz, ny, nx)) < 0.03
     px = np.where(bad, 1.0e10, px)                                              # stencil out of range
     px = np.where((np.arange(nx) % 41 == 40)[None, None, :], 1 - 40.0, px)      # left of the source
+    # stencils on the first columns, rows and planes of the source, and on its last ones
+    lo = np.zeros((nz, ny, nx), bool)
+    lo[0, 0, np.arange(nx) % 37 == 11] = True
+    hi = np.zeros((nz, ny, nx), bool)
+    hi[nz - 1, ny - 1, np.arange(nx) % 37 == 13] = True
+    first = lambda l0: l0 + 1.3 if l0 + 1 >= 0 else l0 + 0.7   # (int) truncates toward zero
+    px = np.where(lo, first(i["lminx"]), np.where(hi, i["lmaxx"] - 1.7, px))
+    py = np.where(lo, first(i["lminy"]), np.where(hi, i["lmaxy"] - 1.7, py))
+    pz = np.where(lo, 1.2, np.where(hi, cfg.nk - 0.5, pz))
     pos = np.stack([px, py, pz], -1).astype(np.float32).reshape(-1)
     xyz = torch.from_numpy(pos).cuda()
     n_model = (cfg.ni + 2 * MH) * (cfg.nj + 2 * MH) * cfg.nk
@@ -347,7 +363,7 @@ def test_shared_columns_edge_positions(T, math):
     A.adz_tricub_hlt(ref, xyz, ctx.ztab(True), *box[:5], box[5], dl, math=M, post=3, src_off=src_off)
     item = dict(geom=ctx.ztab(True), xyz=xyz, box=box, dlayout=dl, fields=A.field_list(stack, src_off, dst_off))
     plan = A.Plan([item], L.ADZ_F32, math=M, post=3,
-                  tune=dict(fields_per_pass=2, points_outer=0, blocks_per_cu=2, xcd_chunks=0, stage=4))
+                  tune=dict(fields_per_pass=2, points_outer=0, blocks_per_cu=2, xcd_chunks=0, stage=stage))
     plan.execute()
     torch.cuda.synchronize()
     nan_seen = False
