@@ -138,9 +138,23 @@ def main():
     dst = [torch.zeros(n_mod, dtype=torch.float32, device=dev) for _ in range(nf)]
     post = [{k: torch.zeros(n_mod, dtype=torch.float32, device=dev) for k in ("lin", "min", "max")}
             for _ in range(nb)]
-    airm1, airm0, pkps, mask = S.airmass(cfg, g, device=dev, mh=MH)
-    airm1, airm0, pkps = airm1.reshape(-1).contiguous(), airm0.reshape(-1).contiguous(), pkps.reshape(-1).contiguous()
+    # pressures at times P and M (nk+1 momentum levels), pr_t and p0 at M: set_post_tr_hlt forms
+    # the air masses and pr_t/p0 from them in every step, as after the pressure adjustment
+    _, _, _, mask = S.airmass(cfg, g, device=dev, mh=MH)
     mask = mask.reshape(-1).contiguous()
+    prng = np.random.default_rng(cfg.seed + 7)
+    shape2 = mod_shape[1:]
+    ps = torch.from_numpy(prng.uniform(0.97, 1.03, shape2)).to(dev)
+    pres_m = [(torch.exp(torch.tensor(g["zm"][1:cfg.nk + 2], device=dev))[:, None, None] * ps[None] *
+               (1.0 + 1e-3 * q)).float().reshape(-1) for q in range(2)]
+    pres_t = (torch.exp(torch.tensor(g["zt"][1:cfg.nk + 1], device=dev))[:, None, None] * ps[None]).float().reshape(-1)
+    pres_0 = (1e5 * ps).float().reshape(-1)
+    cell_area = ((S.RAYT * g["h"]) ** 2 * torch.ones(shape2, dtype=torch.float64, device=dev)).reshape(-1)
+    del ps
+    airm1, airm0, pkps = (torch.zeros(n_mod, device=dev) for _ in range(3))
+
+    def air_masses():  # set_post_tr_hlt.F90:53-70
+        A.adz_set_post_tr(ctx, mod_layout, pres_m[0], pres_m[1], pres_t, pres_0, cell_area, airm1, airm0, pkps)
     deficit = torch.zeros(max(ntr, 1), dtype=torch.float64, device=dev)
     fmass = torch.zeros(2 * max(ntr, 1), dtype=torch.float64, device=dev)  # (mass_fo, mass_fi) per tracer
     lam_flux = not args.no_lam_flux and g["bc_lam_flux"] == 1
@@ -216,13 +230,15 @@ def main():
         fm = fmass[2 * (ids[0] - 4):2 * (ids[-1] - 3)] if fx else None
         bc(airm1, airm0, mask, deficit[sl], pkps=pkps, flux=fm)
 
-    def step():
+    def step():  # dynstep.F90:94-98: tracers interpolated, pressure adjusted, air masses, fixed
         step_traj()
         planA.execute()
+        for ids, plan, bc in groups:
+            plan.execute()
+        air_masses()
         if fx:
             fx(pb, airm0, mask, fmass)
         for ids, plan, bc in groups:
-            plan.execute()
             fixer(ids, bc)
 
     ev = lambda: torch.cuda.Event(enable_timing=True)
@@ -299,6 +315,7 @@ def main():
     t_plan0 = timed(plan0.execute, 5) if plan0 else float("nan")
     t_bc = sum(timed(lambda: fixer(i, bc), 2) for i, _, bc in groups)
     t_flux = timed(lambda: fx(pb, airm0, mask, fmass), 3) if fx else 0.0
+    t_air = timed(air_masses, 3)
     flux_active = fx.active() if fx else 0
     # the trajectory with the other storage, beside it
     other = L.ADZ_F64 if pos_dt == L.ADZ_F32 else L.ADZ_F32
@@ -310,7 +327,7 @@ def main():
     ctx2.free()
     log(f"phases: traject {t_traj:.2f} ms ({args.traj_storage} storage; {t_traj2:.2f} ms with the other), "
         f"interpolation {t_interp:.2f} ms (tracer-group kernel {t_plan0:.2f} ms), Bermejo-Conde {t_bc:.2f} ms, "
-        f"LAM flux {t_flux:.2f} ms ({flux_active} of {npts_b} targets interpolated)")
+        f"LAM flux {t_flux:.2f} ms ({flux_active} of {npts_b} targets interpolated), air masses {t_air:.2f} ms")
 
     # ILMC (mono = 2) on one tracer group's cubic results, beside the step (not part of it:
     # the workload's tracers are fixed by Bermejo-Conde).  Two inputs: the group interpolated
@@ -486,7 +503,7 @@ def main():
         "phases_ms": {"traject": round(t_traj, 3),
                       "traject_" + ("f64" if args.traj_storage == "f32" else "f32") + "_storage": round(t_traj2, 3),
                       "interpolation": round(t_interp, 3),
-                      "bermejo_conde": round(t_bc, 3), "lam_flux": round(t_flux, 3),
+                      "bermejo_conde": round(t_bc, 3), "lam_flux": round(t_flux, 3), "air_masses": round(t_air, 3),
                       "eager_step": round(eager_ms, 3)},
         "lam_flux_targets_interpolated": flux_active,
         "ilmc_beside_step": ilmc,

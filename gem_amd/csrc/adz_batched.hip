@@ -1077,6 +1077,24 @@ __global__ __launch_bounds__(kTile) void k_hlt(DevItem it, InlineFields F) {
                                              nullptr, 2, 0);
 }
 
+// The same for FP32 sources with LDS-staged boxes (staging mode 5): every field of the call
+// in one pass per tile, tiles in super-columns of kHltOrder tile rows, and block b on tile
+// (b % 8) * per + b / 8, so that each XCD (blocks go round-robin over the 8 XCDs) walks one
+// contiguous eighth of the tile list and its L2 sees neighbouring tiles.
+constexpr int kHltOrder = 3;
+
+template <class TP, int MATH, int POST>
+__global__ __launch_bounds__(kTile, MATH == ADZ_MATH_F32 ? 4 : 1) void k_hlt5(DevItem it, InlineFields F,
+                                                                               long ntiles) {
+  __shared__ float boxA[kMaxRows * kPitch];
+  __shared__ float boxB[kMaxRows * kPitch];
+  __shared__ int red[6 * kTJ];
+  const long per = (ntiles + 7) / 8;
+  const long t = (long)(blockIdx.x % 8) * per + blockIdx.x / 8;
+  if (t >= ntiles) return;  // the whole block leaves: no barrier is reached
+  dma_tile<MATH, POST, TP>(it, F, t, 0, it.nf, boxA, boxB, red, kHltOrder);
+}
+
 }  // namespace adz
 
 using namespace adz;
@@ -1166,6 +1184,14 @@ void* pick_plan(int post, int stage) {
 }
 template <class TF, class TP, int M>
 void* pick_hlt(int post) {
+  if constexpr (std::is_same<TF, float>::value) {
+    switch (post) {
+      case 0: return (void*)&k_hlt5<TP, M, 0>;
+      case 1: return (void*)&k_hlt5<TP, M, 1>;
+      case 2: return (void*)&k_hlt5<TP, M, 2>;
+      default: return (void*)&k_hlt5<TP, M, 3>;
+    }
+  }
   switch (post) {
     case 0: return (void*)&k_hlt<TF, TP, M, 0>;
     case 1: return (void*)&k_hlt<TF, TP, M, 1>;
@@ -1238,8 +1264,15 @@ int adz_tricub_hlt(const adz_field* fld, int nptr, const void* xyz, int pdtype, 
       const adz_field& a = fld[f0 + f];
       F.src[f] = a.src; F.dst[f] = a.dst; F.lin[f] = a.lin; F.mn[f] = a.mn; F.mx[f] = a.mx;
     }
-    void* args[] = {&it, &F};
-    ADZ_HIP_CHECK(hipLaunchKernel(kern, dim3((unsigned)it.ptiles), dim3(kTile), args, 0, (hipStream_t)stream));
+    if (fdtype == ADZ_F32) {  // k_hlt5: tiles in super-columns, an eighth of the list per XCD
+      long ntiles = item_ptiles(it.nx, it.ny, it.nz, kHltOrder);
+      long grid = 8 * ((ntiles + 7) / 8);
+      void* args[] = {&it, &F, &ntiles};
+      ADZ_HIP_CHECK(hipLaunchKernel(kern, dim3((unsigned)grid), dim3(kTile), args, 0, (hipStream_t)stream));
+    } else {
+      void* args[] = {&it, &F};
+      ADZ_HIP_CHECK(hipLaunchKernel(kern, dim3((unsigned)it.ptiles), dim3(kTile), args, 0, (hipStream_t)stream));
+    }
   }
   return ADZ_OK;
 }

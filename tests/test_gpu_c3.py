@@ -106,3 +106,36 @@ def test_departure_cell_indices_bit_exact(C3):
     assert np.array_equal(ijk.cpu().numpy(), rijk)
     assert np.array_equal(zl.cpu().numpy() != 0, rzl != 0)
     torch.cuda.synchronize()
+
+
+def test_momentum_and_thermo_fields_fp32_error_vs_fp64_oracle(C3):
+    """The dynamics fields of c3 on their own position sets and descriptors, FP32 storage and
+    arithmetic against the FP64 oracle: u on Adz_pmu and v on Adz_pmv (momentum levels),
+    rhsc on Adz_pm, T on Adz_pt (thermodynamic levels), extended-layout sources as
+    adz_main.F90:38-60 reads them.  Max and RMS relative error <= 1e-5 per field."""
+    torch, A, L = C3["torch"], C3["A"], C3["L"]
+    cfg, g, ctx = C3["cfg"], C3["g"], C3["ctx"]
+    i = ctx.info
+    nit, njt = i["nit"], i["njt"]
+    ext_off = (1 - i["lminx"]) + nit * (1 - i["lminy"])
+    mod = C3["mod"]
+    n_mod = int(np.prod(mod))
+    ext = [f.numpy() for f in S.fields(cfg, 4, seed=cfg.seed + 1)]  # smooth u, v, T, rhsc
+    groups = [("u", ext[0], A.ADZ_A_PMU, False, (i["i0u"], i["inu"], i["j0"], i["jn"], i["k0"])),
+              ("v", ext[1], A.ADZ_A_PMV, False, (i["i0"], i["in"], i["j0v"], i["jnv"], i["k0"])),
+              ("rhsc", ext[3], A.ADZ_A_PM, False, (i["i0"], i["in"], i["j0"], i["jn"], i["k0"])),
+              ("T", ext[2], A.ADZ_A_PT, True, (i["i0"], i["in"], i["j0"], i["jn"], i["k0t"]))]
+    for name, src, which, thermo, box in groups:
+        xyz = ctx.array(which)
+        stack = [dict(src=torch.from_numpy(src).float().cuda(), dst=torch.zeros(n_mod, device="cuda"))]
+        A.adz_tricub_hlt(stack, xyz, ctx.ztab(thermo), *box, cfg.nk, C3["dl"], ext=True, math=L.ADZ_MATH_F32,
+                         src_off=ext_off)
+        torch.cuda.synchronize()
+        lev = (g["zt"] if thermo else g["zm"])[1:cfg.nk + 1]
+        exp = O.zyx1_m_n(O.Ztab(lev, nit, njt, 0, 0), [src], ext_off, xyz.cpu().numpy(), O.ORD_AVX2)[0]
+        i0, in_, j0, jn, k0 = box
+        got = stack[0]["dst"].reshape(mod)[k0 - 1:, j0 - 1 + MH:jn + MH, i0 - 1 + MH:in_ + MH]
+        got = got.double().cpu().numpy().reshape(-1)
+        mx, rms = _errs(got, exp)
+        print(f"[c3] fp32 storage, {name}: max rel {mx:.3e}, rms rel {rms:.3e}", flush=True)
+        assert rms <= 1e-5 and mx <= 1e-5, (name, mx, rms)
