@@ -180,11 +180,22 @@ __device__ __forceinline__ void store_nan(const Fields& F, int f, long o) {
   if (POST & ADZ_POST_STORE) { ((typename GPtr<TF>::type)F.l(f))[o] = nan; ((typename GPtr<TF>::type)F.n(f))[o] = nan; ((typename GPtr<TF>::type)F.x(f))[o] = nan; }
 }
 
+// Minimum over the 64 lanes of a wave (every lane active), returned wave-uniform: lane
+// pairs, quads, half-rows and rows of 16 by DPP moves within each row (xor 1, xor 2, the
+// half-row and row mirrors), then the four row minima read as scalars.  No LDS traffic.
+__device__ __forceinline__ int wave_min(int v) {
+  v = min(v, __builtin_amdgcn_update_dpp(v, v, 0xB1, 0xF, 0xF, false));   // quad_perm [1,0,3,2]
+  v = min(v, __builtin_amdgcn_update_dpp(v, v, 0x4E, 0xF, 0xF, false));   // quad_perm [2,3,0,1]
+  v = min(v, __builtin_amdgcn_update_dpp(v, v, 0x141, 0xF, 0xF, false));  // row_half_mirror
+  v = min(v, __builtin_amdgcn_update_dpp(v, v, 0x140, 0xF, 0xF, false));  // row_mirror
+  return min(min(__builtin_amdgcn_readlane(v, 0), __builtin_amdgcn_readlane(v, 16)),
+             min(__builtin_amdgcn_readlane(v, 32), __builtin_amdgcn_readlane(v, 48)));
+}
+
 // block-wide minimum of 6 values over the tile's kTJ waves of 64; `red` = 6 * kTJ ints of LDS
 __device__ __forceinline__ void block_min6(int* v, int* red) {
 #pragma unroll
-  for (int q = 0; q < 6; q++)
-    for (int s = 32; s > 0; s >>= 1) v[q] = min(v[q], __shfl_xor(v[q], s, 64));
+  for (int q = 0; q < 6; q++) v[q] = wave_min(v[q]);
   if ((threadIdx.x & 63) == 0)
 #pragma unroll
     for (int q = 0; q < 6; q++) red[q * kTJ + (threadIdx.x >> 6)] = v[q];
@@ -265,23 +276,29 @@ __device__ __forceinline__ void per_lane(const DevItem& it, const Fields& F, int
 struct TileXY {
   int ib, jb, kk;
 };
-__device__ __forceinline__ TileXY tile_at(const DevItem& it, long ptile, int level_inner, int tw = kTI) {
-  const long nib = (it.nx + tw - 1) / tw, njb = (it.ny + kTJ - 1) / kTJ;
+// (32-bit arithmetic: the host refuses items of 2^31 tiles or more)
+__device__ __forceinline__ TileXY tile_at(const DevItem& it, long ptile_l, int level_inner, int tw = kTI) {
+  const unsigned ptile = (unsigned)ptile_l;
+  const unsigned nib = (unsigned)(it.nx + tw - 1) / (unsigned)tw, njb = (unsigned)(it.ny + kTJ - 1) / kTJ;
+  const unsigned nz = (unsigned)it.nz;
   TileXY c;
   if (level_inner >= 2) {
-    const long S = level_inner, per = (long)it.nz * S;
-    const long sc = ptile / per, r = ptile % per;
-    c.kk = (int)(r / S);
-    c.ib = (int)(sc % nib);
-    c.jb = (int)((sc / nib) * S + r % S);
+    const unsigned S = (unsigned)level_inner, per = nz * S;
+    const unsigned sc = ptile / per, r = ptile - sc * per;
+    const unsigned kq = r / S, scj = sc / nib;
+    c.kk = (int)kq;
+    c.ib = (int)(sc - scj * nib);
+    c.jb = (int)(scj * S + (r - kq * S));
   } else if (level_inner) {
-    c.kk = (int)(ptile % it.nz);
-    c.ib = (int)((ptile / it.nz) % nib);
-    c.jb = (int)(ptile / (it.nz * nib));
+    const unsigned q = ptile / nz, qj = q / nib;
+    c.kk = (int)(ptile - q * nz);
+    c.ib = (int)(q - qj * nib);
+    c.jb = (int)qj;
   } else {
-    c.ib = (int)(ptile % nib);
-    c.jb = (int)((ptile / nib) % njb);
-    c.kk = (int)(ptile / (nib * njb));
+    const unsigned q = ptile / nib, qk = q / njb;
+    c.ib = (int)(ptile - q * nib);
+    c.jb = (int)(q - qk * njb);
+    c.kk = (int)qk;
   }
   return c;
 }
@@ -1028,10 +1045,16 @@ __device__ __forceinline__ void plan_loop(const PlanArgs& A, TF* boxA, TF* boxB,
 #pragma unroll
       for (int e = 0; e < (int)(sizeof(DevItem) / 8); e++) d[e] = w[e];
     }
-    long local = t - it.tile0;
+    const long local = t - it.tile0;
     long ptile, grp;
-    if (A.points_outer) { ptile = local / it.ngroups; grp = local % it.ngroups; }
-    else { grp = local / it.ptiles; ptile = local % it.ptiles; }
+    if (local < (1L << 31)) {  // 32-bit division (the usual case)
+      const unsigned lo32 = (unsigned)local;
+      if (A.points_outer) { const unsigned q = lo32 / (unsigned)it.ngroups; ptile = q; grp = lo32 - q * (unsigned)it.ngroups; }
+      else { const unsigned q = lo32 / (unsigned)it.ptiles; grp = q; ptile = lo32 - q * (unsigned)it.ptiles; }
+    } else {
+      if (A.points_outer) { ptile = local / it.ngroups; grp = local % it.ngroups; }
+      else { grp = local / it.ptiles; ptile = local % it.ptiles; }
+    }
     int f0 = (int)grp * A.g, f1 = min(it.nf, f0 + A.g);
     ItemFields F{it.src, it.dst, it.lin, it.mn, it.mx};
     if constexpr (STAGE == 4 && std::is_same<TF, float>::value)
@@ -1220,8 +1243,8 @@ void* pick(int fdt, int pdt, int math, int post, int stage = 2) {
 
 // targets per tile along i: kSW for the shared-column staging of FP32 sources, else kTI
 // built-in tile parameters: LDS-staged boxes (stage 5; other field types fall back to
-// per-lane loads) over super-columns of three tile rows, four fields per pass
-adz_tune default_tune() { return adz_tune{4, 0, 4, 1, 5, 3}; }
+// per-lane loads) over super-columns of three tile rows, ten fields per pass
+adz_tune default_tune() { return adz_tune{10, 0, 4, 1, 5, 3}; }
 
 int tile_width(int stage, int fdtype) { return (stage == 4 && fdtype == ADZ_F32) ? kSW : kTI; }
 
@@ -1258,6 +1281,8 @@ int adz_tricub_hlt(const adz_field* fld, int nptr, const void* xyz, int pdtype, 
     DevItem it;
     if (!fill_item(&it, geom, xyz, i0, in, j0, jn, k0, nk, dminx, dmaxx, dminy, dmaxy, ext, nf))
       return fail(ADZ_EINVAL, "adz_tricub_hlt: bad box or descriptor");
+    if (item_ptiles(it.nx, it.ny, it.nz, kHltOrder) >= (1L << 31) - 8)
+      return fail(ADZ_EINVAL, "adz_tricub_hlt: a box of 2^31 tiles or more");
     InlineFields F;
     std::memset(&F, 0, sizeof(F));
     for (int f = 0; f < nf; f++) {
@@ -1322,6 +1347,7 @@ int adz_plan_create(const adz_item* items, int nitems, int fdtype, int math, int
     }
     dv[q].ngroups = (a.nptr + T.fields_per_pass - 1) / T.fields_per_pass;
     dv[q].ptiles = item_ptiles(dv[q].nx, dv[q].ny, dv[q].nz, T.level_inner, tile_width(T.stage, fdtype));
+    if (dv[q].ptiles >= (1L << 31)) return fail(ADZ_EINVAL, "adz_plan_create: an item of 2^31 tiles or more");
     dv[q].tile0 = tiles;
     tiles += dv[q].ptiles * dv[q].ngroups;
     dv[q].src = (const void* const*)(intptr_t)(0 * nptr_total + pofs);  // patched below
@@ -1429,7 +1455,8 @@ int adz_plan_autotune(const adz_item* items, int nitems, int fdtype, int math, i
   cur.fields_per_pass = std::min(cur.fields_per_pass, maxf);
   float cur_ms = 0.f;
   int rc = time_one(cur, &cur_ms);
-  const int stages[] = {1, 2, 3, 4, 5}, fpps[] = {2, 4, 8, 16, 32, 64}, bpcs[] = {2, 4, 8, 16}, flags[] = {0, 1},
+  const int stages[] = {1, 2, 3, 4, 5}, fpps[] = {2, 3, 4, 5, 8, 10, 16, 20, 32, 40, 64}, bpcs[] = {2, 4, 8, 16},
+            flags[] = {0, 1},
             orders[] = {0, 1, 2, 3, 4, 6, 8};
   const int nst = fdtype == ADZ_F32 ? 5 : 2, nor = 7;
   // staging and tile order together first: the LDS-staged forms depend on the L2 reuse the
@@ -1450,13 +1477,13 @@ int adz_plan_autotune(const adz_item* items, int nitems, int fdtype, int math, i
   for (int ci = 0; ci < 5 && rc == ADZ_OK; ci++) {
     const int coord = coords[ci];
     const int* vals = coord == 1 ? orders : coord == 2 ? fpps : coord == 3 ? bpcs : flags;
-    int nv = coord == 1 ? nor : coord == 2 ? 6 : coord == 3 ? 4 : 2;
+    int nv = coord == 1 ? nor : coord == 2 ? 11 : coord == 3 ? 4 : 2;
     for (int v = 0; v < nv && rc == ADZ_OK; v++) {
       adz_tune t = cur;
       int* slot = coord == 1 ? &t.level_inner : coord == 2 ? &t.fields_per_pass
                 : coord == 3 ? &t.blocks_per_cu : coord == 4 ? &t.points_outer : &t.xcd_chunks;
       if (*slot == vals[v]) continue;
-      if (coord == 2 && vals[v] > maxf && vals[v - 1] >= maxf) continue;  // one candidate covers all fields
+      if (coord == 2 && v > 0 && vals[v] > maxf && vals[v - 1] >= maxf) continue;  // one candidate covers all fields
       *slot = coord == 2 ? std::min(vals[v], maxf) : vals[v];
       if (coord == 2 && t.fields_per_pass == cur.fields_per_pass) continue;
       float ms = 0.f;

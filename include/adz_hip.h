@@ -227,7 +227,7 @@ typedef struct adz_item {
 } adz_item;
 
 /* Tile parameters of the persistent kernel; zero fields_per_pass / blocks_per_cu take the
-   built-in default {4, 0, 4, 1, 5, 3}, and stage 0 takes the built-in staging together
+   built-in default {10, 0, 4, 1, 5, 3}, and stage 0 takes the built-in staging together
    with its level_inner; adz_plan_autotune searches them on the actual plan. */
 typedef struct adz_tune {
   int fields_per_pass; /* fields interpolated per pass over the points (1..64)          */
