@@ -146,15 +146,16 @@ __global__ __launch_bounds__(256) void k_traject(S* __restrict__ pxyzm, double* 
                                                   const S* __restrict__ uarr, const S* __restrict__ varr,
                                                   const S* __restrict__ warr, ZtabDev lv, TrajParams P, int niter,
                                                   double dtA, double dtD, double dtzA, double dtzD) {
-  long a = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  long nxy = (long)P.ni * P.nj;
-  long npts = nxy * (P.nk - P.k00 + 1);
-  if (a >= npts) return;
-  int k = (int)(a / nxy) + P.k00;
-  int j = (int)((a % nxy) / P.ni) + 1;
-  int i = (int)(a % P.ni) + 1;
+  // 32-bit point index (adz_set refuses tiles of 2^31 points or more)
+  const unsigned a = blockIdx.x * blockDim.x + threadIdx.x;
+  const unsigned nxy = (unsigned)P.ni * (unsigned)P.nj;
+  if (a >= nxy * (unsigned)(P.nk - P.k00 + 1)) return;
+  const unsigned kq = a / nxy, rem = a - kq * nxy, jq = rem / (unsigned)P.ni;
+  const int k = (int)kq + P.k00;
+  const int j = (int)jq + 1;
+  const int i = (int)(rem - jq * (unsigned)P.ni) + 1;
   long p = p3_idx(P, i, j, k);
-  long aw = (long)(i - 1) + (long)P.ni * (j - 1) + nxy * (k - 1);
+  long aw = (long)(i - 1) + (long)P.ni * (j - 1) + (long)nxy * (k - 1);
   S px = pxyzm[p], py = pxyzm[p + 1], pz = pxyzm[p + 2];
   S ud = (S)0, vd = (S)0, wd = (S)0;
   double xm = 0.0, ym = 0.0, pzm = 0.0;
@@ -405,6 +406,8 @@ int adz_set(const adz_grid* g, int pos_dtype, int itraj, adz_ctx** out) {
   if (pos_dtype != ADZ_F32 && pos_dtype != ADZ_F64) return fail(ADZ_EINVAL, "adz_set: pos_dtype must be F32 or F64");
   if (g->nk < 4 || g->ni < 8 || g->nj < 8 || g->halox < 2 || g->haloy < 2 || itraj < 1)
     return fail(ADZ_EINVAL, "adz_set: need nk>=4, ni,nj>=8, halo>=2, itraj>=1");
+  if ((long)g->ni * g->nj * (g->nk + 1) >= (1L << 31))
+    return fail(ADZ_EINVAL, "adz_set: a tile of 2^31 points or more (split the domain)");
   adz_ctx* c = new adz_ctx();
   c->pos_dtype = pos_dtype; c->itraj = itraj; c->niter = itraj;
   TrajParams& P = c->P;
