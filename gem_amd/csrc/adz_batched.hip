@@ -192,19 +192,20 @@ __device__ __forceinline__ int wave_min(int v) {
              min(__builtin_amdgcn_readlane(v, 32), __builtin_amdgcn_readlane(v, 48)));
 }
 
-// block-wide minimum of 6 values over the tile's kTJ waves of 64; `red` = 6 * kTJ ints of LDS
+// block-wide minimum of 6 values over the tile's TH waves of 64; `red` = 6 * TH ints of LDS
+template <int TH = kTJ>
 __device__ __forceinline__ void block_min6(int* v, int* red) {
 #pragma unroll
   for (int q = 0; q < 6; q++) v[q] = wave_min(v[q]);
   if ((threadIdx.x & 63) == 0)
 #pragma unroll
-    for (int q = 0; q < 6; q++) red[q * kTJ + (threadIdx.x >> 6)] = v[q];
+    for (int q = 0; q < 6; q++) red[q * TH + (threadIdx.x >> 6)] = v[q];
   __syncthreads();
 #pragma unroll
   for (int q = 0; q < 6; q++) {
-    int m = red[q * kTJ];
+    int m = red[q * TH];
 #pragma unroll
-    for (int w = 1; w < kTJ; w++) m = min(m, red[q * kTJ + w]);
+    for (int w = 1; w < TH; w++) m = min(m, red[q * TH + w]);
     v[q] = m;
   }
   __syncthreads();
@@ -277,9 +278,10 @@ struct TileXY {
   int ib, jb, kk;
 };
 // (32-bit arithmetic: the host refuses items of 2^31 tiles or more)
-__device__ __forceinline__ TileXY tile_at(const DevItem& it, long ptile_l, int level_inner, int tw = kTI) {
+__device__ __forceinline__ TileXY tile_at(const DevItem& it, long ptile_l, int level_inner, int tw = kTI,
+                                         int th = kTJ) {
   const unsigned ptile = (unsigned)ptile_l;
-  const unsigned nib = (unsigned)(it.nx + tw - 1) / (unsigned)tw, njb = (unsigned)(it.ny + kTJ - 1) / kTJ;
+  const unsigned nib = (unsigned)(it.nx + tw - 1) / (unsigned)tw, njb = (unsigned)(it.ny + th - 1) / (unsigned)th;
   const unsigned nz = (unsigned)it.nz;
   TileXY c;
   if (level_inner >= 2) {
@@ -309,8 +311,8 @@ __device__ __forceinline__ void tile_coords(const DevItem& it, long ptile, int l
 }
 
 // tiles of an item in the order above
-inline long item_ptiles(int nx, int ny, int nz, int level_inner, int tw = kTI) {
-  const long nib = (nx + tw - 1) / tw, njb = (ny + kTJ - 1) / kTJ;
+inline long item_ptiles(int nx, int ny, int nz, int level_inner, int tw = kTI, int th = kTJ) {
+  const long nib = (nx + tw - 1) / tw, njb = (ny + th - 1) / th;
   if (level_inner >= 2) return nib * ((njb + level_inner - 1) / level_inner) * level_inner * nz;
   return nib * njb * nz;
 }
@@ -882,19 +884,25 @@ __device__ __forceinline__ void shared_tile(const DevItem& it, const Fields& F, 
 // evaluated; one block barrier per field.  Tiles whose box does not fit (more than
 // kMaxRows5 rows or 77 columns), rows not a multiple of 4 floats apart, or sources beyond
 // 32-bit buffer offsets use per-lane loads.  The arithmetic and its order are those of the
-// per-lane path, so both give the same bits.
+// per-lane path, so both give the same bits.  Staging mode 6 is the same with tiles of 64 x 8
+// targets and blocks of eight waves: a box of 8 target rows spans ~12 source rows against
+// ~7 for 4, so about a fifth fewer DMA bytes per target, and the tile's fixed work (box
+// extents, DMA offsets) is shared by twice the targets.
 constexpr int kQRow = 20;                               // 16-byte quads per LDS row
 constexpr int kPitch5 = 4 * kQRow;                      // floats per LDS row
 constexpr int kMaxRows5 = kMaxRows * kPitch / kPitch5;  // rows per buffer (the stage-1 buffers)
-constexpr int kDmaMax = 5;                              // DMA instructions per wave and field
-static_assert(4 * kDmaMax * 64 >= kMaxRows5 * kQRow, "DMA slots cover a full buffer");
+constexpr int kTJ6 = 8;                                 // target rows of a staging-mode-6 tile
+template <int TH> constexpr int dma_max() { return (kMaxRows5 * kQRow + 64 * TH - 1) / (64 * TH); }
+static_assert(4 * dma_max<4>() * 64 >= kMaxRows5 * kQRow && 8 * dma_max<8>() * 64 >= kMaxRows5 * kQRow,
+              "DMA slots cover a full buffer");
 
-template <int MATH, int POST, class TP, class Fields>
+template <int MATH, int POST, class TP, int TH, class Fields>
 __device__ __forceinline__ void dma_tile(const DevItem& it, const Fields& F, long ptile, int f0, int f1,
                                          float* boxA, float* boxB, int* red, int level_inner) {
+  constexpr int kDmaMax = dma_max<TH>();  // DMA instructions per wave and field
   const int lane = threadIdx.x & (kTI - 1), wv = threadIdx.x / kTI;
-  const TileXY tc = tile_at(it, ptile, level_inner);
-  const int ii = tc.ib * kTI + lane, jj = tc.jb * kTJ + wv;
+  const TileXY tc = tile_at(it, ptile, level_inner, kTI, TH);
+  const int ii = tc.ib * kTI + lane, jj = tc.jb * TH + wv;
   const bool valid = ii < it.nx && jj < it.ny;
   const int i = it.i0 + ii, j = it.j0 + jj, k = it.k0 + tc.kk;
   const long p = (long)ii + (long)it.nx * (jj + (long)it.ny * tc.kk);
@@ -914,7 +922,7 @@ __device__ __forceinline__ void dma_tile(const DevItem& it, const Fields& F, lon
   const int big = 1 << 30;
   int ext[6] = {ok ? x.ix - 1 : big, ok ? -(x.ix + 2) : big, ok ? x.iy - 1 : big, ok ? -(x.iy + 2) : big,
                 ok ? x.pz : big, ok ? -(x.pz + (x.zlinear ? 1 : 3)) : big};
-  block_min6(ext, red);
+  block_min6<TH>(ext, red);
 #pragma unroll
   for (int q = 0; q < 6; q++) ext[q] = __builtin_amdgcn_readfirstlane(ext[q]);  // block-uniform
   const int x0 = ext[0], y0 = ext[2], z0 = ext[4];
@@ -937,7 +945,7 @@ __device__ __forceinline__ void dma_tile(const DevItem& it, const Fields& F, lon
     return;
   }
   // this lane's DMA slots: instruction q of wave wv fills the 64 quads g*64 .. g*64+63 of
-  // the buffer (g = wv + 4 q), quad Q = row Q / kQRow, column quad Q % kQRow; element
+  // the buffer (g = wv + TH q), quad Q = row Q / kQRow, column quad Q % kQRow; element
   // offsets from minoff of the lane's quad (before the field's shift)
   // (32-bit arithmetic: every offset is below span < 2^31 bytes; R / nyb by a float
   // reciprocal, exact since R < 64 and nyb < 64 keep (R + 0.5) / nyb >= 1/128 from an integer)
@@ -949,7 +957,7 @@ __device__ __forceinline__ void dma_tile(const DevItem& it, const Fields& F, lon
   unsigned act = 0;
 #pragma unroll
   for (int q = 0; q < kDmaMax; q++) {
-    const int Q = (wv + 4 * q) * 64 + lane;
+    const int Q = (wv + TH * q) * 64 + lane;
     const int R = Q / kQRow, cq = Q - R * kQRow;
     const int zz = (int)(((float)R + 0.5f) * rnyb), yy = R - zz * nyb;
     rel[q] = rel0 + 4 * cq + yy * ni32 + zz * nij32;
@@ -976,7 +984,7 @@ __device__ __forceinline__ void dma_tile(const DevItem& it, const Fields& F, lon
     for (int q = 0; q < kDmaMax; q++) {
       if (act & (1u << q))
         __builtin_amdgcn_raw_ptr_buffer_load_lds(
-            rs, (__attribute__((address_space(3))) void*)(buf + (wv + 4 * q) * 256), 16,
+            rs, (__attribute__((address_space(3))) void*)(buf + (wv + TH * q) * 256), 16,
             (rel[q] - sh) * 4, 0, 0, 0);
     }
   };
@@ -1060,7 +1068,9 @@ __device__ __forceinline__ void plan_loop(const PlanArgs& A, TF* boxA, TF* boxB,
     if constexpr (STAGE == 4 && std::is_same<TF, float>::value)
       shared_tile<MATH, POST, TP>(it, F, ptile, f0, f1, A.level_inner);
     else if constexpr (STAGE == 5 && std::is_same<TF, float>::value)
-      dma_tile<MATH, POST, TP>(it, F, ptile, f0, f1, (float*)boxA, (float*)boxB, red, A.level_inner);
+      dma_tile<MATH, POST, TP, kTJ>(it, F, ptile, f0, f1, (float*)boxA, (float*)boxB, red, A.level_inner);
+    else if constexpr (STAGE == 6 && std::is_same<TF, float>::value)
+      dma_tile<MATH, POST, TP, kTJ6>(it, F, ptile, f0, f1, (float*)boxA, (float*)boxB, red, A.level_inner);
     else
       run_tile<TF, TP, MATH, POST, ItemFields>(it, F, ptile, f0, f1, boxA, boxB, red, rowoff, STAGE, A.level_inner);
   }
@@ -1082,15 +1092,16 @@ __global__ __launch_bounds__(kTile) void k_plan(PlanArgs A) {
   }
 }
 
-// Staging mode 5 (FP32 sources): its own kernel.  In the FP32 arithmetic it is held to 128
-// VGPRs, so that four blocks per CU (the number its two LDS buffers allow) keep four waves
-// on every SIMD; the reference arithmetic needs more registers than that without spilling.
-template <class TP, int MATH, int POST>
-__global__ __launch_bounds__(kTile, MATH == ADZ_MATH_F32 ? 4 : 1) void k_plan5(PlanArgs A) {
+// Staging modes 5 and 6 (FP32 sources, tiles of TH = 4 or 8 target rows, one wave per row):
+// their own kernel.  In the FP32 arithmetic it is held to 128 VGPRs, so that the blocks its
+// two LDS buffers allow per CU keep four waves on every SIMD; the reference arithmetic needs
+// more registers than that without spilling.
+template <class TP, int MATH, int POST, int TH>
+__global__ __launch_bounds__(64 * TH, MATH == ADZ_MATH_F32 ? 4 : 1) void k_plan5(PlanArgs A) {
   __shared__ float boxA[kMaxRows * kPitch];
   __shared__ float boxB[kMaxRows * kPitch];
-  __shared__ int red[6 * kTJ];
-  plan_loop<float, TP, MATH, POST, 5>(A, boxA, boxB, red, nullptr);
+  __shared__ int red[6 * TH];
+  plan_loop<float, TP, MATH, POST, TH == kTJ ? 5 : 6>(A, boxA, boxB, red, nullptr);
 }
 
 // One item with inline field pointers (the adz_tricub_hlt entry point).
@@ -1115,7 +1126,7 @@ __global__ __launch_bounds__(kTile, MATH == ADZ_MATH_F32 ? 4 : 1) void k_hlt5(De
   const long per = (ntiles + 7) / 8;
   const long t = (long)(blockIdx.x % 8) * per + blockIdx.x / 8;
   if (t >= ntiles) return;  // the whole block leaves: no barrier is reached
-  dma_tile<MATH, POST, TP>(it, F, t, 0, it.nf, boxA, boxB, red, kHltOrder);
+  dma_tile<MATH, POST, TP, kTJ>(it, F, t, 0, it.nf, boxA, boxB, red, kHltOrder);
 }
 
 }  // namespace adz
@@ -1175,10 +1186,18 @@ void* pick_plan(int post, int stage) {
     }
     if (stage == 5) {
       switch (post) {
-        case 0: return (void*)&k_plan5<TP, M, 0>;
-        case 1: return (void*)&k_plan5<TP, M, 1>;
-        case 2: return (void*)&k_plan5<TP, M, 2>;
-        default: return (void*)&k_plan5<TP, M, 3>;
+        case 0: return (void*)&k_plan5<TP, M, 0, kTJ>;
+        case 1: return (void*)&k_plan5<TP, M, 1, kTJ>;
+        case 2: return (void*)&k_plan5<TP, M, 2, kTJ>;
+        default: return (void*)&k_plan5<TP, M, 3, kTJ>;
+      }
+    }
+    if (stage == 6) {
+      switch (post) {
+        case 0: return (void*)&k_plan5<TP, M, 0, kTJ6>;
+        case 1: return (void*)&k_plan5<TP, M, 1, kTJ6>;
+        case 2: return (void*)&k_plan5<TP, M, 2, kTJ6>;
+        default: return (void*)&k_plan5<TP, M, 3, kTJ6>;
       }
     }
     if (stage == 3) {
@@ -1247,6 +1266,7 @@ void* pick(int fdt, int pdt, int math, int post, int stage = 2) {
 adz_tune default_tune() { return adz_tune{10, 0, 4, 1, 5, 3}; }
 
 int tile_width(int stage, int fdtype) { return (stage == 4 && fdtype == ADZ_F32) ? kSW : kTI; }
+int tile_height(int stage, int fdtype) { return (stage == 6 && fdtype == ADZ_F32) ? kTJ6 : kTJ; }
 
 int device_cus() {
   static int cus = 0;
@@ -1310,7 +1330,7 @@ int adz_plan_create(const adz_item* items, int nitems, int fdtype, int math, int
   for (int q = 0; q < nitems; q++)
     if (items[q].pdtype != pdt) return fail(ADZ_EINVAL, "adz_plan_create: items must share the position dtype");
   if (!pick<true>(fdtype, pdt, math, post)) return fail(ADZ_EINVAL, "adz_plan_create: unsupported dtype/math");
-  if (tune && (tune->stage < 0 || tune->stage > 5)) return fail(ADZ_EINVAL, "adz_plan_create: stage must be 0..5");
+  if (tune && (tune->stage < 0 || tune->stage > 6)) return fail(ADZ_EINVAL, "adz_plan_create: stage must be 0..6");
   if (tune && (tune->xcd_chunks < 0 || tune->xcd_chunks > 1))
     return fail(ADZ_EINVAL, "adz_plan_create: xcd_chunks must be 0 or 1");
   adz_tune T = default_tune();
@@ -1346,7 +1366,8 @@ int adz_plan_create(const adz_item* items, int nitems, int fdtype, int math, int
       ptrs[4 * nptr_total + pofs + f] = a.fld[f].mx;
     }
     dv[q].ngroups = (a.nptr + T.fields_per_pass - 1) / T.fields_per_pass;
-    dv[q].ptiles = item_ptiles(dv[q].nx, dv[q].ny, dv[q].nz, T.level_inner, tile_width(T.stage, fdtype));
+    dv[q].ptiles = item_ptiles(dv[q].nx, dv[q].ny, dv[q].nz, T.level_inner, tile_width(T.stage, fdtype),
+                               tile_height(T.stage, fdtype));
     if (dv[q].ptiles >= (1L << 31)) return fail(ADZ_EINVAL, "adz_plan_create: an item of 2^31 tiles or more");
     dv[q].tile0 = tiles;
     tiles += dv[q].ptiles * dv[q].ngroups;
@@ -1392,7 +1413,8 @@ int adz_plan_execute(const adz_plan* p, adz_stream stream) {
   void* kern = pick<true>(p->fdtype, p->pdtype, p->math, p->post, p->tune.stage);
   PlanArgs a = p->args;
   void* args[] = {&a};
-  const unsigned nthr = (p->tune.stage == 3 && p->fdtype == ADZ_F32) ? kPCThreads : kTile;
+  const unsigned nthr = p->fdtype != ADZ_F32 ? kTile
+                        : p->tune.stage == 3 ? kPCThreads : p->tune.stage == 6 ? 64 * kTJ6 : kTile;
   ADZ_HIP_CHECK(hipLaunchKernel(kern, dim3(p->grid), dim3(nthr), args, 0, (hipStream_t)stream));
   return ADZ_OK;
 }
@@ -1455,10 +1477,10 @@ int adz_plan_autotune(const adz_item* items, int nitems, int fdtype, int math, i
   cur.fields_per_pass = std::min(cur.fields_per_pass, maxf);
   float cur_ms = 0.f;
   int rc = time_one(cur, &cur_ms);
-  const int stages[] = {1, 2, 3, 4, 5}, fpps[] = {2, 3, 4, 5, 8, 10, 16, 20, 32, 40, 64}, bpcs[] = {2, 4, 8, 16},
+  const int stages[] = {1, 2, 3, 4, 5, 6}, fpps[] = {2, 3, 4, 5, 8, 10, 16, 20, 32, 40, 64}, bpcs[] = {2, 4, 8, 16},
             flags[] = {0, 1},
             orders[] = {0, 1, 2, 3, 4, 6, 8};
-  const int nst = fdtype == ADZ_F32 ? 5 : 2, nor = 7;
+  const int nst = fdtype == ADZ_F32 ? 6 : 2, nor = 7;
   // staging and tile order together first: the LDS-staged forms depend on the L2 reuse the
   // tile order gives, so one is not searched at the other's best value
   for (int s = 0; s < nst && rc == ADZ_OK; s++)

@@ -244,7 +244,8 @@ typedef struct adz_tune {
                           tiles); 5 (FP32 sources): all four waves copy each field's
                           box into LDS with 16-byte DMA, double-buffered, and every
                           lane evaluates from LDS (the fastest form at c5 with a
-                          level_inner of 3 to 6)                                        */
+                          level_inner of 3 to 6); 6 (FP32 sources): the same with
+                          tiles of 64 x 8 targets and blocks of eight waves             */
   int level_inner;     /* tile order. 0: i, j, then k; 1: consecutive tiles walk the
                           levels of one tile column; S >= 2: super-columns of S tile
                           rows walked level by level (source planes and rows reused

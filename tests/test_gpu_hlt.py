@@ -200,7 +200,10 @@ def _plan_case(T):
                                        level_inner=4),
                                   dict(fields_per_pass=5, points_outer=0, blocks_per_cu=4, xcd_chunks=1, stage=5),
                                   dict(fields_per_pass=2, points_outer=1, blocks_per_cu=2, xcd_chunks=0, stage=5,
-                                       level_inner=3)])
+                                       level_inner=3),
+                                  dict(fields_per_pass=5, points_outer=0, blocks_per_cu=4, xcd_chunks=1, stage=6,
+                                       level_inner=3),
+                                  dict(fields_per_pass=1, points_outer=1, blocks_per_cu=2, xcd_chunks=0, stage=6)])
 def test_plan_matches_single_calls(T, tune):
     torch, L, A = T
     items, refs, keep = _plan_case(T)
@@ -293,7 +296,9 @@ def test_shared_columns_match_single_calls(T, post, math, pdt):
                  dict(fields_per_pass=3, points_outer=1, blocks_per_cu=2, xcd_chunks=0, stage=4, level_inner=1),
                  dict(fields_per_pass=1, points_outer=0, blocks_per_cu=3, xcd_chunks=1, stage=4, level_inner=3),
                  dict(fields_per_pass=3, points_outer=0, blocks_per_cu=4, xcd_chunks=1, stage=5),
-                 dict(fields_per_pass=1, points_outer=1, blocks_per_cu=2, xcd_chunks=0, stage=5, level_inner=2)):
+                 dict(fields_per_pass=1, points_outer=1, blocks_per_cu=2, xcd_chunks=0, stage=5, level_inner=2),
+                 dict(fields_per_pass=3, points_outer=0, blocks_per_cu=4, xcd_chunks=1, stage=6, level_inner=3),
+                 dict(fields_per_pass=2, points_outer=1, blocks_per_cu=2, xcd_chunks=0, stage=6)):
         for stack, _ in refs:
             for a in stack:
                 for k in ("dst", "lin", "min", "max"):
@@ -313,10 +318,10 @@ def test_shared_columns_match_single_calls(T, post, math, pdt):
         c.free()
 
 
-@pytest.mark.parametrize("stage", [4, 5])
+@pytest.mark.parametrize("stage", [4, 5, 6])
 @pytest.mark.parametrize("math", ["f32", "ref"])
 def test_shared_columns_edge_positions(T, math, stage):
-    """Staging modes 4 and 5 with positions that break lane sharing and box staging on
+    """Staging modes 4, 5 and 6 with positions that break lane sharing and box staging on
     purpose: stencils out of range (NaN targets whose lanes serve no neighbour), positions
     on cell faces, a cell repeated or skipped every few targets, levels clamped at both
     ends, stencils on the first columns of the source.  Bit for bit the single-call result
