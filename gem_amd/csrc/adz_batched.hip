@@ -1111,22 +1111,23 @@ __global__ __launch_bounds__(kTile) void k_hlt(DevItem it, InlineFields F) {
                                              nullptr, 2, 0);
 }
 
-// The same for FP32 sources with LDS-staged boxes (staging mode 5): every field of the call
-// in one pass per tile, tiles in super-columns of kHltOrder tile rows, and block b on tile
+// The same for FP32 sources with LDS-staged boxes (staging mode 6: tiles of 64 x 8 targets,
+// blocks of eight waves): every field of the call in one pass per tile, tiles in
+// super-columns of kHltOrder tile rows, and block b on tile
 // (b % 8) * per + b / 8, so that each XCD (blocks go round-robin over the 8 XCDs) walks one
 // contiguous eighth of the tile list and its L2 sees neighbouring tiles.
 constexpr int kHltOrder = 3;
 
 template <class TP, int MATH, int POST>
-__global__ __launch_bounds__(kTile, MATH == ADZ_MATH_F32 ? 4 : 1) void k_hlt5(DevItem it, InlineFields F,
-                                                                               long ntiles) {
+__global__ __launch_bounds__(64 * kTJ6, MATH == ADZ_MATH_F32 ? 4 : 1) void k_hlt5(DevItem it, InlineFields F,
+                                                                                   long ntiles) {
   __shared__ float boxA[kMaxRows * kPitch];
   __shared__ float boxB[kMaxRows * kPitch];
-  __shared__ int red[6 * kTJ];
+  __shared__ int red[6 * kTJ6];
   const long per = (ntiles + 7) / 8;
   const long t = (long)(blockIdx.x % 8) * per + blockIdx.x / 8;
   if (t >= ntiles) return;  // the whole block leaves: no barrier is reached
-  dma_tile<MATH, POST, TP, kTJ>(it, F, t, 0, it.nf, boxA, boxB, red, kHltOrder);
+  dma_tile<MATH, POST, TP, kTJ6>(it, F, t, 0, it.nf, boxA, boxB, red, kHltOrder);
 }
 
 }  // namespace adz
@@ -1310,10 +1311,10 @@ int adz_tricub_hlt(const adz_field* fld, int nptr, const void* xyz, int pdtype, 
       F.src[f] = a.src; F.dst[f] = a.dst; F.lin[f] = a.lin; F.mn[f] = a.mn; F.mx[f] = a.mx;
     }
     if (fdtype == ADZ_F32) {  // k_hlt5: tiles in super-columns, an eighth of the list per XCD
-      long ntiles = item_ptiles(it.nx, it.ny, it.nz, kHltOrder);
+      long ntiles = item_ptiles(it.nx, it.ny, it.nz, kHltOrder, kTI, kTJ6);
       long grid = 8 * ((ntiles + 7) / 8);
       void* args[] = {&it, &F, &ntiles};
-      ADZ_HIP_CHECK(hipLaunchKernel(kern, dim3((unsigned)grid), dim3(kTile), args, 0, (hipStream_t)stream));
+      ADZ_HIP_CHECK(hipLaunchKernel(kern, dim3((unsigned)grid), dim3(64 * kTJ6), args, 0, (hipStream_t)stream));
     } else {
       void* args[] = {&it, &F};
       ADZ_HIP_CHECK(hipLaunchKernel(kern, dim3((unsigned)it.ptiles), dim3(kTile), args, 0, (hipStream_t)stream));
