@@ -841,8 +841,9 @@ def pmc_summary(args):
 def plan_kernel_name(tune, pos, post):
     """the batched kernel a plan with these tiles launches for FP32 fields and arithmetic"""
     stage = (tune or {}).get("stage") or 5
-    if stage in (5, 6):
-        return f"k_plan5<{pos},F32,{post},{4 if stage == 5 else 8}>"
+    if stage in (5, 6, 7, 8):
+        rows, levels = {5: (4, 1), 6: (8, 1), 7: (4, 2), 8: (4, 4)}[stage]
+        return f"k_plan5<{pos},F32,{post},{rows},{levels}>"
     if stage == 3:
         return f"k_plan_pc<{pos},F32,{post}>"
     return f"k_plan<float,{pos},F32,{post},stage {stage}>"

@@ -278,11 +278,12 @@ struct TileXY {
   int ib, jb, kk;
 };
 // (32-bit arithmetic: the host refuses items of 2^31 tiles or more)
+// Tiles of tl > 1 levels: kk is the tile's level block (levels kk * tl .. kk * tl + tl - 1).
 __device__ __forceinline__ TileXY tile_at(const DevItem& it, long ptile_l, int level_inner, int tw = kTI,
-                                         int th = kTJ) {
+                                         int th = kTJ, int tl = 1) {
   const unsigned ptile = (unsigned)ptile_l;
   const unsigned nib = (unsigned)(it.nx + tw - 1) / (unsigned)tw, njb = (unsigned)(it.ny + th - 1) / (unsigned)th;
-  const unsigned nz = (unsigned)it.nz;
+  const unsigned nz = (unsigned)(it.nz + tl - 1) / (unsigned)tl;
   TileXY c;
   if (level_inner >= 2) {
     const unsigned S = (unsigned)level_inner, per = nz * S;
@@ -311,8 +312,8 @@ __device__ __forceinline__ void tile_coords(const DevItem& it, long ptile, int l
 }
 
 // tiles of an item in the order above
-inline long item_ptiles(int nx, int ny, int nz, int level_inner, int tw = kTI, int th = kTJ) {
-  const long nib = (nx + tw - 1) / tw, njb = (ny + th - 1) / th;
+inline long item_ptiles(int nx, int ny, int nz_levels, int level_inner, int tw = kTI, int th = kTJ, int tl = 1) {
+  const long nib = (nx + tw - 1) / tw, njb = (ny + th - 1) / th, nz = (nz_levels + tl - 1) / tl;
   if (level_inner >= 2) return nib * ((njb + level_inner - 1) / level_inner) * level_inner * nz;
   return nib * njb * nz;
 }
@@ -887,25 +888,38 @@ __device__ __forceinline__ void shared_tile(const DevItem& it, const Fields& F, 
 // per-lane path, so both give the same bits.  Staging mode 6 is the same with tiles of 64 x 8
 // targets and blocks of eight waves: a box of 8 target rows spans ~12 source rows against
 // ~7 for 4, so about a fifth fewer DMA bytes per target, and the tile's fixed work (box
-// extents, DMA offsets) is shared by twice the targets.
+// extents, DMA offsets) is shared by twice the targets.  Staging modes 7 and 8 take tiles of
+// 64 x 4 targets over 2 and 4 consecutive levels (blocks of 8 and 16 waves, one wave per
+// target row and level).  A source plane is read by the stencils of about four levels, so
+// the planes of a box dominate its volume: at c5, a box over 64 x 4 x 4 targets holds ~7
+// rows of ~7 planes against ~11 rows of 4 planes for 64 x 8 x 1, about 14 against 25 DMA
+// bytes per target.
 constexpr int kQRow = 20;                               // 16-byte quads per LDS row
 constexpr int kPitch5 = 4 * kQRow;                      // floats per LDS row
 constexpr int kMaxRows5 = kMaxRows * kPitch / kPitch5;  // rows per buffer (the stage-1 buffers)
 constexpr int kTJ6 = 8;                                 // target rows of a staging-mode-6 tile
-template <int TH> constexpr int dma_max() { return (kMaxRows5 * kQRow + 64 * TH - 1) / (64 * TH); }
-static_assert(4 * dma_max<4>() * 64 >= kMaxRows5 * kQRow && 8 * dma_max<8>() * 64 >= kMaxRows5 * kQRow,
+constexpr int kMaxRows8 = 96;                           // rows per buffer of the 16-wave blocks
+// rows per LDS buffer for blocks of W waves: 16-wave blocks run one per CU and take larger
+// buffers (2 x 30 KB); the others keep the stage-1 buffer size, two blocks per CU
+template <int W> constexpr int box_rows() { return W >= 16 ? kMaxRows8 : kMaxRows5; }
+template <int W> constexpr int dma_max() { return (box_rows<W>() * kQRow + 64 * W - 1) / (64 * W); }
+static_assert(4 * dma_max<4>() * 64 >= kMaxRows5 * kQRow && 8 * dma_max<8>() * 64 >= kMaxRows5 * kQRow &&
+                  16 * dma_max<16>() * 64 >= kMaxRows8 * kQRow && dma_max<4>() <= 32,
               "DMA slots cover a full buffer");
+static_assert(kMaxRows5 < 128 && kMaxRows8 < 128, "row split by a float reciprocal below");
 
-template <int MATH, int POST, class TP, int TH, class Fields>
+template <int MATH, int POST, class TP, int TH, int TL, class Fields>
 __device__ __forceinline__ void dma_tile(const DevItem& it, const Fields& F, long ptile, int f0, int f1,
                                          float* boxA, float* boxB, int* red, int level_inner) {
-  constexpr int kDmaMax = dma_max<TH>();  // DMA instructions per wave and field
+  constexpr int W = TH * TL;                  // waves of the block: one per target row and level
+  constexpr int kRows = box_rows<W>();
+  constexpr int kDmaMax = dma_max<W>();       // DMA instructions per wave and field
   const int lane = threadIdx.x & (kTI - 1), wv = threadIdx.x / kTI;
-  const TileXY tc = tile_at(it, ptile, level_inner, kTI, TH);
-  const int ii = tc.ib * kTI + lane, jj = tc.jb * TH + wv;
-  const bool valid = ii < it.nx && jj < it.ny;
-  const int i = it.i0 + ii, j = it.j0 + jj, k = it.k0 + tc.kk;
-  const long p = (long)ii + (long)it.nx * (jj + (long)it.ny * tc.kk);
+  const TileXY tc = tile_at(it, ptile, level_inner, kTI, TH, TL);
+  const int ii = tc.ib * kTI + lane, jj = tc.jb * TH + wv % TH, kk = tc.kk * TL + wv / TH;
+  const bool valid = ii < it.nx && jj < it.ny && kk < it.nz;
+  const int i = it.i0 + ii, j = it.j0 + jj, k = it.k0 + kk;
+  const long p = (long)ii + (long)it.nx * (jj + (long)it.ny * kk);
   const long o = (long)(i - 1) + (long)it.dni * ((j - 1) + (long)it.dnj * (k - 1));
   Index x;
   bool ok = false;
@@ -922,7 +936,7 @@ __device__ __forceinline__ void dma_tile(const DevItem& it, const Fields& F, lon
   const int big = 1 << 30;
   int ext[6] = {ok ? x.ix - 1 : big, ok ? -(x.ix + 2) : big, ok ? x.iy - 1 : big, ok ? -(x.iy + 2) : big,
                 ok ? x.pz : big, ok ? -(x.pz + (x.zlinear ? 1 : 3)) : big};
-  block_min6<TH>(ext, red);
+  block_min6<W>(ext, red);
 #pragma unroll
   for (int q = 0; q < 6; q++) ext[q] = __builtin_amdgcn_readfirstlane(ext[q]);  // block-uniform
   const int x0 = ext[0], y0 = ext[2], z0 = ext[4];
@@ -939,16 +953,17 @@ __device__ __forceinline__ void dma_tile(const DevItem& it, const Fields& F, lon
   const long first = (long)(x0 - 1 - 3) + (long)(y0 - 1) * gni + (long)z0 * gnij - it.minoff;
   const long last = (long)(x0 - 1) + 4L * cpr - 1 + (long)(y0 + nyb - 2) * gni + (long)(z0 + nzb - 1) * gnij -
                     it.minoff;
-  if (nrows > kMaxRows5 || cpr > kQRow || !it.span || (gni & 3) || (gnij & 3) || first < 0 ||
+  if (nrows > kRows || cpr > kQRow || !it.span || (gni & 3) || (gnij & 3) || first < 0 ||
       (last + 1) * 4 > (long)it.span) {
     per_lane<float, MATH, POST>(it, F, f0, f1, x, ok, valid, cw, o);
     return;
   }
   // this lane's DMA slots: instruction q of wave wv fills the 64 quads g*64 .. g*64+63 of
-  // the buffer (g = wv + TH q), quad Q = row Q / kQRow, column quad Q % kQRow; element
+  // the buffer (g = wv + W q), quad Q = row Q / kQRow, column quad Q % kQRow; element
   // offsets from minoff of the lane's quad (before the field's shift)
   // (32-bit arithmetic: every offset is below span < 2^31 bytes; R / nyb by a float
-  // reciprocal, exact since R < 64 and nyb < 64 keep (R + 0.5) / nyb >= 1/128 from an integer)
+  // reciprocal, exact since R < 128 and nyb < 128 keep (R + 0.5) / nyb >= 1/256 from an
+  // integer, far above the float product's error; slots with R >= nrows are inactive)
   const int rel0 = (int)__builtin_amdgcn_readfirstlane((int)((long)(x0 - 1) + (long)(y0 - 1) * gni +
                                                              (long)z0 * gnij - it.minoff));
   const int ni32 = (int)gni, nij32 = (int)gnij;
@@ -957,7 +972,7 @@ __device__ __forceinline__ void dma_tile(const DevItem& it, const Fields& F, lon
   unsigned act = 0;
 #pragma unroll
   for (int q = 0; q < kDmaMax; q++) {
-    const int Q = (wv + TH * q) * 64 + lane;
+    const int Q = (wv + W * q) * 64 + lane;
     const int R = Q / kQRow, cq = Q - R * kQRow;
     const int zz = (int)(((float)R + 0.5f) * rnyb), yy = R - zz * nyb;
     rel[q] = rel0 + 4 * cq + yy * ni32 + zz * nij32;
@@ -984,7 +999,7 @@ __device__ __forceinline__ void dma_tile(const DevItem& it, const Fields& F, lon
     for (int q = 0; q < kDmaMax; q++) {
       if (act & (1u << q))
         __builtin_amdgcn_raw_ptr_buffer_load_lds(
-            rs, (__attribute__((address_space(3))) void*)(buf + (wv + TH * q) * 256), 16,
+            rs, (__attribute__((address_space(3))) void*)(buf + (wv + W * q) * 256), 16,
             (rel[q] - sh) * 4, 0, 0, 0);
     }
   };
@@ -1068,9 +1083,13 @@ __device__ __forceinline__ void plan_loop(const PlanArgs& A, TF* boxA, TF* boxB,
     if constexpr (STAGE == 4 && std::is_same<TF, float>::value)
       shared_tile<MATH, POST, TP>(it, F, ptile, f0, f1, A.level_inner);
     else if constexpr (STAGE == 5 && std::is_same<TF, float>::value)
-      dma_tile<MATH, POST, TP, kTJ>(it, F, ptile, f0, f1, (float*)boxA, (float*)boxB, red, A.level_inner);
+      dma_tile<MATH, POST, TP, kTJ, 1>(it, F, ptile, f0, f1, (float*)boxA, (float*)boxB, red, A.level_inner);
     else if constexpr (STAGE == 6 && std::is_same<TF, float>::value)
-      dma_tile<MATH, POST, TP, kTJ6>(it, F, ptile, f0, f1, (float*)boxA, (float*)boxB, red, A.level_inner);
+      dma_tile<MATH, POST, TP, kTJ6, 1>(it, F, ptile, f0, f1, (float*)boxA, (float*)boxB, red, A.level_inner);
+    else if constexpr (STAGE == 7 && std::is_same<TF, float>::value)
+      dma_tile<MATH, POST, TP, kTJ, 2>(it, F, ptile, f0, f1, (float*)boxA, (float*)boxB, red, A.level_inner);
+    else if constexpr (STAGE == 8 && std::is_same<TF, float>::value)
+      dma_tile<MATH, POST, TP, kTJ, 4>(it, F, ptile, f0, f1, (float*)boxA, (float*)boxB, red, A.level_inner);
     else
       run_tile<TF, TP, MATH, POST, ItemFields>(it, F, ptile, f0, f1, boxA, boxB, red, rowoff, STAGE, A.level_inner);
   }
@@ -1092,16 +1111,17 @@ __global__ __launch_bounds__(kTile) void k_plan(PlanArgs A) {
   }
 }
 
-// Staging modes 5 and 6 (FP32 sources, tiles of TH = 4 or 8 target rows, one wave per row):
-// their own kernel.  In the FP32 arithmetic it is held to 128 VGPRs, so that the blocks its
-// two LDS buffers allow per CU keep four waves on every SIMD; the reference arithmetic needs
-// more registers than that without spilling.
-template <class TP, int MATH, int POST, int TH>
-__global__ __launch_bounds__(64 * TH, MATH == ADZ_MATH_F32 ? 4 : 1) void k_plan5(PlanArgs A) {
-  __shared__ float boxA[kMaxRows * kPitch];
-  __shared__ float boxB[kMaxRows * kPitch];
-  __shared__ int red[6 * TH];
-  plan_loop<float, TP, MATH, POST, TH == kTJ ? 5 : 6>(A, boxA, boxB, red, nullptr);
+// Staging modes 5 to 8 (FP32 sources, tiles of TH target rows x TL levels, one wave per row
+// and level): their own kernel.  In the FP32 arithmetic it is held to 128 VGPRs, so that the
+// blocks its two LDS buffers allow per CU keep four waves on every SIMD; the reference
+// arithmetic needs more registers than that without spilling (and has no 16-wave form).
+template <int TH, int TL> constexpr int dma_stage() { return TL == 4 ? 8 : TL == 2 ? 7 : TH == kTJ ? 5 : 6; }
+template <class TP, int MATH, int POST, int TH, int TL>
+__global__ __launch_bounds__(64 * TH * TL, MATH == ADZ_MATH_F32 ? 4 : 1) void k_plan5(PlanArgs A) {
+  __shared__ float boxA[box_rows<TH * TL>() * kPitch5];
+  __shared__ float boxB[box_rows<TH * TL>() * kPitch5];
+  __shared__ int red[6 * TH * TL];
+  plan_loop<float, TP, MATH, POST, dma_stage<TH, TL>()>(A, boxA, boxB, red, nullptr);
 }
 
 // One item with inline field pointers (the adz_tricub_hlt entry point).
@@ -1115,19 +1135,23 @@ __global__ __launch_bounds__(kTile) void k_hlt(DevItem it, InlineFields F) {
 // blocks of eight waves): every field of the call in one pass per tile, tiles in
 // super-columns of kHltOrder tile rows, and block b on tile
 // (b % 8) * per + b / 8, so that each XCD (blocks go round-robin over the 8 XCDs) walks one
-// contiguous eighth of the tile list and its L2 sees neighbouring tiles.
+// contiguous eighth of the tile list and its L2 sees neighbouring tiles.  One tile per
+// block: without lin/min/max the kernel needs 72 VGPRs, so seven waves share each SIMD.
+// (The 16-wave form of 64 x 4 x 4 tiles moves fewer bytes, but one such block is resident per
+// CU and its per-field DMA waits are not covered: at c5 with 16 fields and no lin/min/max it
+// gave 150-154 against 200 G point-fields/s.)
 constexpr int kHltOrder = 3;
 
 template <class TP, int MATH, int POST>
 __global__ __launch_bounds__(64 * kTJ6, MATH == ADZ_MATH_F32 ? 4 : 1) void k_hlt5(DevItem it, InlineFields F,
                                                                                    long ntiles) {
-  __shared__ float boxA[kMaxRows * kPitch];
-  __shared__ float boxB[kMaxRows * kPitch];
+  __shared__ float boxA[box_rows<kTJ6>() * kPitch5];
+  __shared__ float boxB[box_rows<kTJ6>() * kPitch5];
   __shared__ int red[6 * kTJ6];
   const long per = (ntiles + 7) / 8;
   const long t = (long)(blockIdx.x % 8) * per + blockIdx.x / 8;
   if (t >= ntiles) return;  // the whole block leaves: no barrier is reached
-  dma_tile<MATH, POST, TP, kTJ6>(it, F, t, 0, it.nf, boxA, boxB, red, kHltOrder);
+  dma_tile<MATH, POST, TP, kTJ6, 1>(it, F, t, 0, it.nf, boxA, boxB, red, kHltOrder);
 }
 
 }  // namespace adz
@@ -1187,18 +1211,36 @@ void* pick_plan(int post, int stage) {
     }
     if (stage == 5) {
       switch (post) {
-        case 0: return (void*)&k_plan5<TP, M, 0, kTJ>;
-        case 1: return (void*)&k_plan5<TP, M, 1, kTJ>;
-        case 2: return (void*)&k_plan5<TP, M, 2, kTJ>;
-        default: return (void*)&k_plan5<TP, M, 3, kTJ>;
+        case 0: return (void*)&k_plan5<TP, M, 0, kTJ, 1>;
+        case 1: return (void*)&k_plan5<TP, M, 1, kTJ, 1>;
+        case 2: return (void*)&k_plan5<TP, M, 2, kTJ, 1>;
+        default: return (void*)&k_plan5<TP, M, 3, kTJ, 1>;
       }
     }
     if (stage == 6) {
       switch (post) {
-        case 0: return (void*)&k_plan5<TP, M, 0, kTJ6>;
-        case 1: return (void*)&k_plan5<TP, M, 1, kTJ6>;
-        case 2: return (void*)&k_plan5<TP, M, 2, kTJ6>;
-        default: return (void*)&k_plan5<TP, M, 3, kTJ6>;
+        case 0: return (void*)&k_plan5<TP, M, 0, kTJ6, 1>;
+        case 1: return (void*)&k_plan5<TP, M, 1, kTJ6, 1>;
+        case 2: return (void*)&k_plan5<TP, M, 2, kTJ6, 1>;
+        default: return (void*)&k_plan5<TP, M, 3, kTJ6, 1>;
+      }
+    }
+    if (stage == 7) {
+      switch (post) {
+        case 0: return (void*)&k_plan5<TP, M, 0, kTJ, 2>;
+        case 1: return (void*)&k_plan5<TP, M, 1, kTJ, 2>;
+        case 2: return (void*)&k_plan5<TP, M, 2, kTJ, 2>;
+        default: return (void*)&k_plan5<TP, M, 3, kTJ, 2>;
+      }
+    }
+    if constexpr (M == ADZ_MATH_F32) {
+      if (stage == 8) {
+        switch (post) {
+          case 0: return (void*)&k_plan5<TP, M, 0, kTJ, 4>;
+          case 1: return (void*)&k_plan5<TP, M, 1, kTJ, 4>;
+          case 2: return (void*)&k_plan5<TP, M, 2, kTJ, 4>;
+          default: return (void*)&k_plan5<TP, M, 3, kTJ, 4>;
+        }
       }
     }
     if (stage == 3) {
@@ -1261,13 +1303,26 @@ void* pick(int fdt, int pdt, int math, int post, int stage = 2) {
   return nullptr;
 }
 
+// Built-in tile parameters, ten fields per pass: for FP32 fields in the FP32 arithmetic,
+// LDS-staged boxes over tiles of 64 x 4 targets x 4 levels (stage 8) with one 16-wave block
+// per CU, the levels of a tile column walked first; otherwise LDS-staged boxes over 64 x 4
+// tiles (stage 5; other field types fall back to per-lane loads) in super-columns of three
+// tile rows.
+adz_tune default_tune(int fdtype, int math) {
+  return fdtype == ADZ_F32 && math == ADZ_MATH_F32 ? adz_tune{10, 0, 1, 1, 8, 1} : adz_tune{10, 0, 4, 1, 5, 3};
+}
 // targets per tile along i: kSW for the shared-column staging of FP32 sources, else kTI
-// built-in tile parameters: LDS-staged boxes (stage 5; other field types fall back to
-// per-lane loads) over super-columns of three tile rows, ten fields per pass
-adz_tune default_tune() { return adz_tune{10, 0, 4, 1, 5, 3}; }
 
 int tile_width(int stage, int fdtype) { return (stage == 4 && fdtype == ADZ_F32) ? kSW : kTI; }
 int tile_height(int stage, int fdtype) { return (stage == 6 && fdtype == ADZ_F32) ? kTJ6 : kTJ; }
+int tile_levels(int stage, int fdtype) {
+  return fdtype != ADZ_F32 ? 1 : stage == 7 ? 2 : stage == 8 ? 4 : 1;
+}
+// threads per block of a plan's kernel
+unsigned plan_threads(int stage, int fdtype) {
+  if (fdtype != ADZ_F32) return kTile;
+  return stage == 3 ? kPCThreads : stage == 6 || stage == 7 ? 512 : stage == 8 ? 1024 : kTile;
+}
 
 int device_cus() {
   static int cus = 0;
@@ -1310,9 +1365,9 @@ int adz_tricub_hlt(const adz_field* fld, int nptr, const void* xyz, int pdtype, 
       const adz_field& a = fld[f0 + f];
       F.src[f] = a.src; F.dst[f] = a.dst; F.lin[f] = a.lin; F.mn[f] = a.mn; F.mx[f] = a.mx;
     }
-    if (fdtype == ADZ_F32) {  // k_hlt5: tiles in super-columns, an eighth of the list per XCD
+    if (fdtype == ADZ_F32) {  // k_hlt5: LDS-staged boxes, an eighth of the tile list per XCD
       long ntiles = item_ptiles(it.nx, it.ny, it.nz, kHltOrder, kTI, kTJ6);
-      long grid = 8 * ((ntiles + 7) / 8);
+      long grid = 8 * ((ntiles + 7) / 8);  // one block per tile
       void* args[] = {&it, &F, &ntiles};
       ADZ_HIP_CHECK(hipLaunchKernel(kern, dim3((unsigned)grid), dim3(64 * kTJ6), args, 0, (hipStream_t)stream));
     } else {
@@ -1331,10 +1386,12 @@ int adz_plan_create(const adz_item* items, int nitems, int fdtype, int math, int
   for (int q = 0; q < nitems; q++)
     if (items[q].pdtype != pdt) return fail(ADZ_EINVAL, "adz_plan_create: items must share the position dtype");
   if (!pick<true>(fdtype, pdt, math, post)) return fail(ADZ_EINVAL, "adz_plan_create: unsupported dtype/math");
-  if (tune && (tune->stage < 0 || tune->stage > 6)) return fail(ADZ_EINVAL, "adz_plan_create: stage must be 0..6");
+  if (tune && (tune->stage < 0 || tune->stage > 8)) return fail(ADZ_EINVAL, "adz_plan_create: stage must be 0..8");
+  if (tune && tune->stage == 8 && fdtype == ADZ_F32 && math != ADZ_MATH_F32)
+    return fail(ADZ_EINVAL, "adz_plan_create: stage 8 needs the FP32 arithmetic");
   if (tune && (tune->xcd_chunks < 0 || tune->xcd_chunks > 1))
     return fail(ADZ_EINVAL, "adz_plan_create: xcd_chunks must be 0 or 1");
-  adz_tune T = default_tune();
+  adz_tune T = default_tune(fdtype, math);
   if (tune) {
     if (tune->fields_per_pass > 0) T.fields_per_pass = std::min(64, tune->fields_per_pass);
     T.points_outer = tune->points_outer;
@@ -1368,7 +1425,7 @@ int adz_plan_create(const adz_item* items, int nitems, int fdtype, int math, int
     }
     dv[q].ngroups = (a.nptr + T.fields_per_pass - 1) / T.fields_per_pass;
     dv[q].ptiles = item_ptiles(dv[q].nx, dv[q].ny, dv[q].nz, T.level_inner, tile_width(T.stage, fdtype),
-                               tile_height(T.stage, fdtype));
+                               tile_height(T.stage, fdtype), tile_levels(T.stage, fdtype));
     if (dv[q].ptiles >= (1L << 31)) return fail(ADZ_EINVAL, "adz_plan_create: an item of 2^31 tiles or more");
     dv[q].tile0 = tiles;
     tiles += dv[q].ptiles * dv[q].ngroups;
@@ -1414,8 +1471,7 @@ int adz_plan_execute(const adz_plan* p, adz_stream stream) {
   void* kern = pick<true>(p->fdtype, p->pdtype, p->math, p->post, p->tune.stage);
   PlanArgs a = p->args;
   void* args[] = {&a};
-  const unsigned nthr = p->fdtype != ADZ_F32 ? kTile
-                        : p->tune.stage == 3 ? kPCThreads : p->tune.stage == 6 ? 64 * kTJ6 : kTile;
+  const unsigned nthr = plan_threads(p->tune.stage, p->fdtype);
   ADZ_HIP_CHECK(hipLaunchKernel(kern, dim3(p->grid), dim3(nthr), args, 0, (hipStream_t)stream));
   return ADZ_OK;
 }
@@ -1474,22 +1530,23 @@ int adz_plan_autotune(const adz_item* items, int nitems, int fdtype, int math, i
     int rf = adz_plan_free(p);
     return rc != ADZ_OK ? rc : rf;
   };
-  adz_tune cur = default_tune();
+  adz_tune cur = default_tune(fdtype, math);
   cur.fields_per_pass = std::min(cur.fields_per_pass, maxf);
   float cur_ms = 0.f;
   int rc = time_one(cur, &cur_ms);
-  const int stages[] = {1, 2, 3, 4, 5, 6}, fpps[] = {2, 3, 4, 5, 8, 10, 16, 20, 32, 40, 64}, bpcs[] = {2, 4, 8, 16},
-            flags[] = {0, 1},
-            orders[] = {0, 1, 2, 3, 4, 6, 8};
-  const int nst = fdtype == ADZ_F32 ? 6 : 2, nor = 7;
+  const int stages[] = {1, 2, 3, 4, 5, 6, 7, 8}, fpps[] = {2, 3, 4, 5, 8, 10, 16, 20, 32, 40, 64},
+            bpcs[] = {1, 2, 4, 8, 16}, flags[] = {0, 1}, orders[] = {0, 1, 2, 3, 4, 6, 8};
+  const int nst = fdtype != ADZ_F32 ? 2 : math == ADZ_MATH_F32 ? 8 : 7, nor = 7;
   // staging and tile order together first: the LDS-staged forms depend on the L2 reuse the
-  // tile order gives, so one is not searched at the other's best value
+  // tile order gives, so one is not searched at the other's best value.  Each staging is
+  // timed at 4 blocks per CU, blocks of 16 waves at 1 (one of them is resident per CU).
   for (int s = 0; s < nst && rc == ADZ_OK; s++)
     for (int v = 0; v < nor && rc == ADZ_OK; v++) {
       adz_tune t = cur;
       t.stage = stages[s];
       t.level_inner = orders[v];
-      if (t.stage == cur.stage && t.level_inner == cur.level_inner) continue;
+      t.blocks_per_cu = plan_threads(t.stage, fdtype) >= 1024 ? 1 : 4;
+      if (t.stage == cur.stage && t.level_inner == cur.level_inner && t.blocks_per_cu == cur.blocks_per_cu) continue;
       float ms = 0.f;
       rc = time_one(t, &ms);
       if (rc == ADZ_OK && ms < cur_ms) { cur = t; cur_ms = ms; }
@@ -1500,7 +1557,7 @@ int adz_plan_autotune(const adz_item* items, int nitems, int fdtype, int math, i
   for (int ci = 0; ci < 5 && rc == ADZ_OK; ci++) {
     const int coord = coords[ci];
     const int* vals = coord == 1 ? orders : coord == 2 ? fpps : coord == 3 ? bpcs : flags;
-    int nv = coord == 1 ? nor : coord == 2 ? 11 : coord == 3 ? 4 : 2;
+    int nv = coord == 1 ? nor : coord == 2 ? 11 : coord == 3 ? 5 : 2;
     for (int v = 0; v < nv && rc == ADZ_OK; v++) {
       adz_tune t = cur;
       int* slot = coord == 1 ? &t.level_inner : coord == 2 ? &t.fields_per_pass

@@ -227,8 +227,9 @@ typedef struct adz_item {
 } adz_item;
 
 /* Tile parameters of the persistent kernel; zero fields_per_pass / blocks_per_cu take the
-   built-in default {10, 0, 4, 1, 5, 3}, and stage 0 takes the built-in staging together
-   with its level_inner; adz_plan_autotune searches them on the actual plan. */
+   built-in default, {10, 0, 1, 1, 8, 1} for FP32 fields in the FP32 arithmetic and
+   {10, 0, 4, 1, 5, 3} otherwise, and stage 0 takes the built-in staging together with its
+   level_inner; adz_plan_autotune searches them on the actual plan. */
 typedef struct adz_tune {
   int fields_per_pass; /* fields interpolated per pass over the points (1..64)          */
   int points_outer;    /* 1: a block runs every field group of a point tile; 0: a pass
@@ -243,9 +244,12 @@ typedef struct adz_tune {
                           neighbours' targets, partials shared by lane shifts (61-wide
                           tiles); 5 (FP32 sources): all four waves copy each field's
                           box into LDS with 16-byte DMA, double-buffered, and every
-                          lane evaluates from LDS (the fastest form at c5 with a
-                          level_inner of 3 to 6); 6 (FP32 sources): the same with
-                          tiles of 64 x 8 targets and blocks of eight waves             */
+                          lane evaluates from LDS; 6 (FP32 sources): the same with
+                          tiles of 64 x 8 targets and blocks of eight waves; 7 and 8
+                          (FP32 sources): tiles of 64 x 4 targets over 2 and 4
+                          consecutive levels, blocks of 8 and 16 waves (8: the FP32
+                          arithmetic only, ADZ_EINVAL otherwise; one block per CU is
+                          resident; the fastest form at c5)                             */
   int level_inner;     /* tile order. 0: i, j, then k; 1: consecutive tiles walk the
                           levels of one tile column; S >= 2: super-columns of S tile
                           rows walked level by level (source planes and rows reused

@@ -203,7 +203,14 @@ def _plan_case(T):
                                        level_inner=3),
                                   dict(fields_per_pass=5, points_outer=0, blocks_per_cu=4, xcd_chunks=1, stage=6,
                                        level_inner=3),
-                                  dict(fields_per_pass=1, points_outer=1, blocks_per_cu=2, xcd_chunks=0, stage=6)])
+                                  dict(fields_per_pass=1, points_outer=1, blocks_per_cu=2, xcd_chunks=0, stage=6),
+                                  dict(fields_per_pass=5, points_outer=0, blocks_per_cu=4, xcd_chunks=1, stage=7,
+                                       level_inner=3),
+                                  dict(fields_per_pass=2, points_outer=1, blocks_per_cu=2, xcd_chunks=0, stage=7),
+                                  dict(fields_per_pass=5, points_outer=0, blocks_per_cu=1, xcd_chunks=1, stage=8,
+                                       level_inner=3),
+                                  dict(fields_per_pass=1, points_outer=1, blocks_per_cu=2, xcd_chunks=0, stage=8,
+                                       level_inner=1)])
 def test_plan_matches_single_calls(T, tune):
     torch, L, A = T
     items, refs, keep = _plan_case(T)
@@ -298,7 +305,15 @@ def test_shared_columns_match_single_calls(T, post, math, pdt):
                  dict(fields_per_pass=3, points_outer=0, blocks_per_cu=4, xcd_chunks=1, stage=5),
                  dict(fields_per_pass=1, points_outer=1, blocks_per_cu=2, xcd_chunks=0, stage=5, level_inner=2),
                  dict(fields_per_pass=3, points_outer=0, blocks_per_cu=4, xcd_chunks=1, stage=6, level_inner=3),
-                 dict(fields_per_pass=2, points_outer=1, blocks_per_cu=2, xcd_chunks=0, stage=6)):
+                 dict(fields_per_pass=2, points_outer=1, blocks_per_cu=2, xcd_chunks=0, stage=6),
+                 dict(fields_per_pass=3, points_outer=0, blocks_per_cu=4, xcd_chunks=1, stage=7, level_inner=3),
+                 dict(fields_per_pass=1, points_outer=1, blocks_per_cu=2, xcd_chunks=0, stage=7, level_inner=1),
+                 dict(fields_per_pass=3, points_outer=0, blocks_per_cu=1, xcd_chunks=1, stage=8, level_inner=2),
+                 dict(fields_per_pass=2, points_outer=1, blocks_per_cu=2, xcd_chunks=0, stage=8)):
+        if tune["stage"] == 8 and M != L.ADZ_MATH_F32:  # the 16-wave form is FP32 arithmetic only
+            with pytest.raises(RuntimeError, match="stage 8"):
+                A.Plan(items, L.ADZ_F32, math=M, post=post, tune=tune)
+            continue
         for stack, _ in refs:
             for a in stack:
                 for k in ("dst", "lin", "min", "max"):
@@ -318,17 +333,19 @@ def test_shared_columns_match_single_calls(T, post, math, pdt):
         c.free()
 
 
-@pytest.mark.parametrize("stage", [4, 5, 6])
+@pytest.mark.parametrize("stage", [4, 5, 6, 7, 8])
 @pytest.mark.parametrize("math", ["f32", "ref"])
 def test_shared_columns_edge_positions(T, math, stage):
-    """Staging modes 4, 5 and 6 with positions that break lane sharing and box staging on
+    """Staging modes 4 to 8 with positions that break lane sharing and box staging on
     purpose: stencils out of range (NaN targets whose lanes serve no neighbour), positions
     on cell faces, a cell repeated or skipped every few targets, levels clamped at both
     ends, stencils on the first columns of the source.  Bit for bit the single-call result
     (NaN patterns included), with lin/min/max."""
     torch, L, A = T
     M = L.ADZ_MATH_F32 if math == "f32" else L.ADZ_MATH_REF
-    cfg = S.Config("e", 150, 26, 10, 4, 300.0, seed=85)
+    if stage == 8 and math == "ref":
+        pytest.skip("staging mode 8 is FP32 arithmetic only (refused, checked in the test above)")
+    cfg = S.Config("e", 150, 26, 11, 4, 300.0, seed=85)   # 11 levels: tiles of 2 and 4 levels ragged
     g, ctx, src64 = _setup(T, cfg, L.ADZ_F32, "f32", nfields=2, seed=cfg.seed)
     i = ctx.info
     box = (i["i0"], i["in"], i["j0"], i["jn"], i["k0t"], cfg.nk)
