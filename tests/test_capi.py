@@ -80,3 +80,28 @@ def test_ilmc_create_refuses_bad_arguments(case):
     assert L.lib().adz_ilmc_create(tr, 1, ctypes.byref(d), ctypes.byref(h)) == 1
     assert "adz_ilmc_create" in L.lib().adz_last_error().decode()
     assert not h.value
+
+
+@pytest.mark.parametrize("case", ["stage9", "stage8_ref", "xcd2"])
+def test_plan_create_refuses_bad_tiles(case):
+    """adz_plan_create checks the tile parameters before it reads an item or touches the
+    device: staging modes run 0..8, staging mode 8 (16-wave blocks) exists for FP32 fields in
+    the FP32 arithmetic only, XCD chunking is 0 or 1."""
+    if not os.path.exists(L.LIB_PATH):
+        pytest.skip("libadz_hip.so not built")
+    from gem_amd import adz as A
+
+    items = (A.AdzItem * 1)()
+    items[0].pdtype = L.ADZ_F32  # the rest is never read: the tile check fails first
+    t = A.AdzTune(fields_per_pass=4, points_outer=0, blocks_per_cu=1, xcd_chunks=1, stage=8, level_inner=1)
+    math, want = L.ADZ_MATH_F32, "stage must be 0..8"
+    if case == "stage9":
+        t.stage = 9
+    elif case == "stage8_ref":
+        math, want = L.ADZ_MATH_REF, "stage 8 needs the FP32 arithmetic"
+    else:
+        t.xcd_chunks, want = 2, "xcd_chunks must be 0 or 1"
+    h = ctypes.c_void_p()
+    assert L.lib().adz_plan_create(items, 1, L.ADZ_F32, math, 0, ctypes.byref(t), ctypes.byref(h)) == 1
+    assert want in L.lib().adz_last_error().decode()
+    assert not h.value
