@@ -388,15 +388,16 @@ def main():
         torch.cuda.empty_cache()
 
     # algorithmic bytes of the tracer-group kernel (one launch): sources + targets + lin/min/max
-    # (3 extra FP32 outputs) + FP64 positions once
+    # (3 extra FP32 outputs) + the positions once, at their storage width
     Nt = npts(tbox)
     ng = len(ids0) if ids0 else 0
-    alg_plan0 = ng * 4 * (n_mod + Nt) + ng * 3 * 4 * Nt + 24 * Nt
+    pos_b = 12 if pos_dt == L.ADZ_F32 else 24
+    alg_plan0 = ng * 4 * (n_mod + Nt) + ng * 3 * 4 * Nt + pos_b * Nt
     achieved = alg_plan0 / (t_plan0 * 1e-3) / 1e9
     traffic = measured_traffic(args.config, tuneB)
     # interpolation-only throughput over all 64 fields
     alg_interp = (3 * 4 * n_ext + (nf - 3) * 4 * n_mod + 4 * pts_total + ntr * 12 * Nt +
-                  24 * (npts(boxes["u"][0]) + npts(boxes["v"][0]) + Nt))
+                  pos_b * (npts(boxes["u"][0]) + npts(boxes["v"][0]) + Nt))
     interp_gpts = pts_total / (t_interp * 1e-3) / 1e9
 
     # one tracer's fixer inputs (after its group's interpolation) for the CPU baseline
@@ -686,6 +687,21 @@ def run_c4(args, np, torch, L, A, S, dev):
         tune, tune_ms = A.Plan.autotune(items, L.ADZ_F32, math=L.ADZ_MATH_F32, reps=2)
         log(f"autotune: {tune} -> {tune_ms:.2f} ms per launch ({time.time() - t0:.1f}s)")
     plan = A.Plan(items, L.ADZ_F32, math=L.ADZ_MATH_F32, tune=tune)
+    pos_b = 12 if args.traj_storage == "f32" else 24  # positions at their storage width
+    alg = sum(ntr * 4 * (st["n_mod"] + npts(st["box"])) + pos_b * npts(st["box"]) for st in state)
+    if args.profile_kernel:
+        for _ in range(5):
+            plan.execute()
+        torch.cuda.synchronize()
+        # calibration copy of known size for the PMC byte counters (1 GiB read + 1 GiB write)
+        x = torch.empty(2 ** 28, dtype=torch.float32, device=dev)
+        y = torch.empty_like(x)
+        y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"profile_kernel": "k_plan (c4, 16 items)", "tiles": tune, "copy_bytes": 2 * 4 * 2 ** 28,
+                          "algorithmic_bytes_per_launch": alg, "kernel_source_sha16": kernel_source_sha()}),
+              flush=True)
+        return
     default_plan = A.Plan(items, L.ADZ_F32, math=L.ADZ_MATH_F32)
 
     def step():
@@ -757,7 +773,13 @@ def run_c4(args, np, torch, L, A, S, dev):
     plain_gpts = pf / (t64 * 1e-3) / 1e9
     log(f"plain FP64 per-field launches: {t64:.1f} ms per step ({plain_gpts:.2f} Gpt/s); FP32 vs FP64 worst "
         f"RMS {worst_rms:.2e}, max {worst_max:.2e} over {nerr} fields")
-    alg = sum(ntr * 4 * (st["n_mod"] + npts(st["box"])) + 24 * npts(st["box"]) for st in state)
+    # the reference CPU interpolator on a bounded sample of the largest subdomain
+    cpu = None
+    if not args.no_cpu:
+        big = state[-1]
+        bc = big["cfg"]
+        cpu = cpu_baseline(np, torch, bc, S.grid_desc(bc), big["src"], big["ctx"].array(A.ADZ_A_PT), big["box"],
+                           (bc.nk, bc.nj + 2 * MH, bc.ni + 2 * MH), ntr + 4, args.cpu_seconds)
     achieved = alg / (t_plan * 1e-3) / 1e9
     out = {
         "metric": "interpolated Gpoints/s + HBM GB/s: FP32 batched vs FP64 plain at 1e-5 RMS",
@@ -778,10 +800,11 @@ def run_c4(args, np, torch, L, A, S, dev):
         "speedup_batched_fp32_vs_fp64_plain": round(t64 / t_plan, 3),
         "error_vs_fp64": {"worst_max_rel": worst_max, "worst_rms_rel": worst_rms, "fields": nerr},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK, 4), "traffic": None,
-                     "kernel": f"{plan_kernel_name(tune, 'double', '0')} over 16 items, {t_plan:.3f} ms/launch, "
+                     "frac": round(achieved / HBM_PEAK, 4), "traffic": measured_traffic("c4", tune),
+                     "kernel": f"{plan_kernel_name(tune, 'float' if pos_b == 12 else 'double', '0')} over 16 items, "
+                               f"{t_plan:.3f} ms/launch, "
                                f"{alg / 1e9:.2f} GB algorithmic"},
-        "cpu_baseline": None,
+        "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
     plan.free()
@@ -789,9 +812,10 @@ def run_c4(args, np, torch, L, A, S, dev):
 
 
 def pmc_summary(args):
-    """profiles/k_plan_<config>_pmc.json: HBM bytes per launch of the tracer-group kernel
-    from rocprofv3 FETCH_SIZE and WRITE_SIZE runs of --profile-kernel, each counter scaled by
-    its reading for the run's closing 1 GiB device copy (1 GiB read, 1 GiB written)."""
+    """profiles/k_plan_<config>_pmc.json: HBM bytes per launch of the batched interpolation
+    kernel (c5: the 20-tracer group; c4: the launch over 16 subdomains) from rocprofv3
+    FETCH_SIZE and WRITE_SIZE runs of --profile-kernel (with the same --config), each counter
+    scaled by its reading for the run's closing 1 GiB device copy (1 GiB read, 1 GiB written)."""
     import csv
     import glob
 
@@ -821,17 +845,24 @@ def pmc_summary(args):
         for r in csv.DictReader(open(st)):
             if "k_plan" in r["Name"]:
                 dur = float(r["AverageNs"])
-    tiles = (tuned_tiles(args.config, args.tiles) or {}).get("tracer_groups")
+    c4 = args.config == "c4"
+    tiles = (tuned_tiles(args.config, args.tiles) or {}).get("plan" if c4 else "tracer_groups")
     out = {"kernel": fplan[0][0],
-           "workload": "c5 tracer group: 20 FP32 tracers (model layout 2052x1028x84) at the thermodynamic departure "
-                       "points of 2048x1024x84 targets, lin/min/max stored",
+           "workload": ("c4: one persistent launch over 16 subdomains (64x48x84 .. 512x384x84), 40 FP32 tracers "
+                        "each (model layout) at their thermodynamic departure points (FP32 positions), no lin/min/max"
+                        if c4 else
+                        "c5 tracer group: 20 FP32 tracers (model layout 2052x1028x84) at the thermodynamic departure "
+                        "points of 2048x1024x84 targets, lin/min/max stored"),
            "tiles": tiles, "kernel_source_sha16": kernel_source_sha(),
            "fetch_size_kib_per_launch": fk, "write_size_kib_per_launch": wk,
            "copy_fetch_size_kib": fcopy[1], "copy_write_size_kib": wcopy[1],
            "fetch_scale": fscale, "write_scale": wscale,
            "hbm_read_bytes_per_launch": int(fk * fscale * 1024), "hbm_write_bytes_per_launch": int(wk * wscale * 1024),
            "hbm_bytes_per_launch": int((fk * fscale + wk * wscale) * 1024),
-           "algorithmic_bytes_per_launch": 73.1e9, "average_launch_ns": dur}
+           # algorithmic bytes: c5, 20 x 4 B x (source + targets) + 20 x 12 B lin/min/max + 12 B FP32
+           # positions per target; c4, the sum over the 16 subdomains of 40 x 4 B x (source + targets) +
+           # 12 B x targets (the workloads as --profile-kernel builds them, FP32 trajectory storage)
+           "algorithmic_bytes_per_launch": 32.41e9 if c4 else 71.04e9, "average_launch_ns": dur}
     path = os.path.join(ROOT, "profiles", f"k_plan_{args.config}_pmc.json")
     with open(path, "w") as f:
         json.dump(out, f, indent=1)
